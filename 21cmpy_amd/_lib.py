@@ -1,0 +1,174 @@
+"""ctypes bindings of lib21cm_hip.so (declared in include/cm21.h).
+
+torch is imported before the library is opened so that the library binds to
+the HIP runtime and rocFFT that torch has already loaded (one runtime per
+process).  There is no CPU fallback: every entry point raises when the library
+is missing or returns an error status.
+"""
+import atexit
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede the CDLL load, see module docstring)
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib21cm_hip.so")
+
+CM21_OK = 0
+STATUS_NAMES = {-1: "CM21_ERR_INVALID", -2: "CM21_ERR_HIP", -3: "CM21_ERR_FFT",
+                -4: "CM21_ERR_DOMAIN", -5: "CM21_ERR_NOMEM"}
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class CM21Error(RuntimeError):
+    """A cm21_* call returned a non-zero status."""
+
+    def __init__(self, func, status, message):
+        self.func, self.status = func, status
+        super().__init__(f"{func}: {STATUS_NAMES.get(status, status)}: {message}")
+
+
+class BubbleParams(ctypes.Structure):
+    """cm21_bubble_params (include/cm21.h)."""
+    _fields_ = [
+        ("box_len", ctypes.c_float),
+        ("deltac", ctypes.c_float),
+        ("zeta", ctypes.c_double),
+        ("growth_inv", ctypes.c_float),
+        ("fc_mean", ctypes.c_float),
+        ("linear_input", ctypes.c_int),
+        ("filter_type", ctypes.c_int),
+        ("ref_fallthrough", ctypes.c_int),
+        ("update", ctypes.c_int),
+        ("sphere_periodic", ctypes.c_int),
+        ("sphere_r_scale", ctypes.c_float),
+    ]
+
+
+VP, I, F, D, SZ, U32 = (ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double,
+                         ctypes.c_size_t, ctypes.c_uint32)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "cm21_abi_version": (I, []),
+    "cm21_last_error": (ctypes.c_char_p, []),
+    "cm21_plan_create": (I, [ctypes.POINTER(VP), I, I]),
+    "cm21_plan_destroy": (I, [VP]),
+    "cm21_r2c": (I, [VP, VP, VP, VP]),
+    "cm21_c2r": (I, [VP, VP, VP, VP]),
+    "cm21_hii_filter": (I, [VP, VP, I, F, I, I, I, F, F, VP]),
+    "cm21_reduce_workspace_bytes": (SZ, []),
+    "cm21_fcoll_sum": (I, [VP, I, F, F, VP, VP, VP]),
+    "cm21_fcoll_threshold": (I, [VP, VP, I, F, F, D, VP, F, I, VP]),
+    "cm21_sphere_workspace_bytes": (SZ, [I]),
+    "cm21_update_sphere": (I, [VP, VP, I, F, F, D, VP, F, F, I, VP, VP]),
+    "cm21_find_bubbles_workspace_bytes": (SZ, [I, I, I]),
+    "cm21_find_bubbles": (I, [VP, VP, VP, c_float_p, c_float_p, I, ctypes.POINTER(BubbleParams), VP, SZ, VP]),
+    "cm21_grf_init": (I, [VP, I, D, D, D, D, U32, U32, VP]),
+    "cm21_subsample": (I, [VP, VP, I, I, F, VP]),
+    "cm21_velocity": (I, [VP, VP, I, F, I, I, F, VP]),
+    "cm21_move_mass": (I, [VP, VP, VP, VP, VP, I, I, F, F, I, VP]),
+    "cm21_delta_T": (I, [VP, VP, VP, I, F, VP, VP, VP]),
+    "cm21_pk_workspace_bytes": (SZ, [I, I]),
+    "cm21_power_spectrum": (I, [VP, VP, VP, I, D, VP, I, I, VP, VP, VP, VP, SZ, VP]),
+}
+
+_lib = None
+
+
+def load():
+    """Open lib21cm_hip.so once and declare its signatures; raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python 21cmpy_amd/build.py` "
+                           "(there is no CPU fallback for the HIP path)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.cm21_abi_version() != 1:
+        raise RuntimeError(f"{LIB_PATH}: unexpected ABI version {lib.cm21_abi_version()}")
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Call a status-returning entry point; raise CM21Error on failure."""
+    lib = load()
+    st = getattr(lib, name)(*args)
+    if st != CM21_OK:
+        msg = lib.cm21_last_error().decode(errors="replace")
+        raise CM21Error(name, st, msg)
+    return st
+
+
+def query(name, *args):
+    """Call a size-returning entry point."""
+    return getattr(load(), name)(*args)
+
+
+def ptr(t):
+    """Device address of a CUDA tensor (checked)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("the cm21 HIP path needs a visible GPU (torch.cuda.is_available() is False)")
+
+
+class Plan:
+    """Owns a cm21_plan (rocFFT r2c/c2r plans for `batch` boxes of side D)."""
+
+    def __init__(self, D, batch=1):
+        require_gpu()
+        h = ctypes.c_void_p()
+        call("cm21_plan_create", ctypes.byref(h), int(D), int(batch))
+        self.handle, self.D, self.batch = h, int(D), int(batch)
+
+    def close(self):
+        if self.handle:
+            load().cm21_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_plans = {}
+
+
+def plan(D, batch=1):
+    """Cached plan per (D, batch) for the current process."""
+    key = (int(D), int(batch))
+    if key not in _plans:
+        _plans[key] = Plan(D, batch)
+    return _plans[key]
+
+
+@atexit.register
+def _release_plans():
+    for p in _plans.values():
+        try:
+            p.close()
+        except Exception:
+            pass
+    _plans.clear()

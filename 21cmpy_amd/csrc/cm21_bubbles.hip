@@ -1,0 +1,328 @@
+// find_HII_bubbles on the GPU: k-space filter, f_coll with its FP64 box sum,
+// threshold / final paint, sphere paint and the radius-loop driver.
+//
+// Reference: Programs/find_bubbles.cu:12-158 (kernels) and
+// Programs/find_HII_bubbles.py:17-193 (loop).  Differences by design:
+//  * the forward transform runs once per box (the reference re-uploads and
+//    re-transforms the same box every radius, find_HII_bubbles.py:141-144);
+//  * the 1/N of the inverse transform and the growth factor of nonlinear
+//    input are folded into the filter multiply;
+//  * the box mean of f_coll is an FP64 device scalar read by the threshold
+//    kernel, so the loop never synchronises with the host.
+#include <cmath>
+
+#include "cm21_internal.hpp"
+#include "cm21_window.hpp"
+
+namespace cm21 {
+int plan_dim(const cm21_plan* p);
+int plan_batch(const cm21_plan* p);
+}
+
+using namespace cm21;
+
+namespace {
+
+__global__ void k_filter(float2* __restrict__ out, const float2* __restrict__ in, int D, size_t Nc,
+                         WindowParams wp, float scale) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= Nc) return;
+    const int H = D / 2 + 1;
+    const int c = (int)(idx % H);
+    const size_t ab = idx / H;
+    const int b = (int)(ab % D);
+    const int a = (int)(ab / D);
+    float w0, w2;
+    bool cut;
+    window_of(kmag_of(a, b, c, D, wp.dk, wp.korder), wp, w0, cut, w2);
+    out[idx] = apply_window(in[idx], w0, cut, w2, scale);
+}
+
+__global__ void k_fill(float* __restrict__ p, size_t n, float v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__device__ inline float fcoll_of(float smoothed, float deltac, float denom) {
+    return erfcf(__fdiv_rn(__fsub_rn(deltac, smoothed), denom));
+}
+
+// Per-block FP64 partial sums of f_coll over a fixed grid of kReduceBlocks.
+__global__ void k_fcoll_partial(const float* __restrict__ sm, size_t N, float deltac, float denom,
+                                double* __restrict__ partial) {
+    double acc = 0.0;
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < N; p += (size_t)gridDim.x * blockDim.x)
+        acc += (double)fcoll_of(sm[p], deltac, denom);
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// Sums `n` partials in a fixed order into *out (one block).
+__global__ void k_sum_partials(const double* __restrict__ partial, int n, double* __restrict__ out) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) *out = acc;
+}
+
+__device__ inline float norm_scale(const double* sum, size_t N, float fc_mean) {
+    if (!sum) return 1.0f;
+    const double mean = *sum / (double)N;
+    return (float)((double)fc_mean / mean);
+}
+
+__global__ void k_threshold(float* __restrict__ ion, const float* __restrict__ sm, size_t N, float deltac,
+                            float denom, double zeta, double inv_zeta, const double* __restrict__ sum,
+                            float fc_mean, int final_step) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const float cur = ion[p];
+    if (cur == 1.0f) return;
+    float f = fcoll_of(sm[p], deltac, denom);
+    if (sum) f = __fmul_rn(f, norm_scale(sum, N, fc_mean));
+    if ((double)f >= inv_zeta)
+        ion[p] = 1.0f;
+    else if (final_step)
+        ion[p] = (float)((double)f * zeta);
+}
+
+// Sphere paint as a separable squared-distance transform.  d2 holds, per
+// cell, a squared distance to the nearest ionizing centre (kFar when none is
+// within range).  Only offsets with o^2 < R^2 matter, because the paint test
+// is d^2 < R^2 (Programs/find_bubbles.cu:97-98).
+constexpr uint32_t kFar = 0x3FFFFFFFu;
+
+__global__ void k_sphere_seed(uint32_t* __restrict__ d2, const float* __restrict__ sm, size_t N, float deltac,
+                              float denom, double inv_zeta, const double* __restrict__ sum, float fc_mean) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    float f = fcoll_of(sm[p], deltac, denom);
+    if (sum) f = __fmul_rn(f, norm_scale(sum, N, fc_mean));
+    d2[p] = ((double)f >= inv_zeta) ? 0u : kFar;
+}
+
+__global__ void k_sphere_pass(uint32_t* __restrict__ out, const uint32_t* __restrict__ in, int D, int axis,
+                              int omax, int periodic) {
+    const size_t N = (size_t)D * D * D;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const size_t stride = axis == 2 ? 1 : (axis == 1 ? (size_t)D : (size_t)D * D);
+    const int pos = (int)((p / stride) % D);
+    const size_t base = p - (size_t)pos * stride;
+    uint32_t best = kFar;
+    for (int o = -omax; o <= omax; ++o) {
+        int q = pos + o;
+        if (periodic) {
+            q %= D;
+            if (q < 0) q += D;
+        } else if (q < 0 || q >= D) {
+            continue;
+        }
+        const uint32_t v = in[base + (size_t)q * stride];
+        if (v < kFar) {
+            const uint32_t c = v + (uint32_t)(o * o);
+            best = c < best ? c : best;
+        }
+    }
+    out[p] = best;
+}
+
+__global__ void k_sphere_apply(float* __restrict__ ion, const uint32_t* __restrict__ d2, size_t N, float R2) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const uint32_t v = d2[p];
+    if (v < kFar && (float)v < R2) ion[p] = 1.0f;
+}
+
+int launch_filter(void* spec_out, const void* spec_in, int D, const WindowParams& wp, float scale,
+                  hipStream_t st) {
+    const size_t Nc = (size_t)D * D * (D / 2 + 1);
+    k_filter<<<grid_for(Nc), kThreads, 0, st>>>((float2*)spec_out, (const float2*)spec_in, D, Nc, wp, scale);
+    CM21_LAUNCH_CHECK("k_filter");
+    return CM21_OK;
+}
+
+int launch_fcoll_sum(const float* sm, int D, float deltac, float denom, double* dev_sum, void* work,
+                     hipStream_t st) {
+    const size_t N = (size_t)D * D * D;
+    double* partial = (double*)work;
+    k_fcoll_partial<<<kReduceBlocks, kThreads, 0, st>>>(sm, N, deltac, denom, partial);
+    CM21_LAUNCH_CHECK("k_fcoll_partial");
+    k_sum_partials<<<1, kThreads, 0, st>>>(partial, kReduceBlocks, dev_sum);
+    CM21_LAUNCH_CHECK("k_sum_partials");
+    return CM21_OK;
+}
+
+int launch_threshold(float* ion, const float* sm, int D, float deltac, float denom, double zeta,
+                     const double* sum, float fc_mean, int final_step, hipStream_t st) {
+    const size_t N = (size_t)D * D * D;
+    k_threshold<<<grid_for(N), kThreads, 0, st>>>(ion, sm, N, deltac, denom, zeta, 1.0 / zeta, sum, fc_mean,
+                                                  final_step);
+    CM21_LAUNCH_CHECK("k_threshold");
+    return CM21_OK;
+}
+
+int launch_sphere(float* ion, const float* sm, int D, float deltac, float denom, double zeta, const double* sum,
+                  float fc_mean, float R_cells, int periodic, void* work, hipStream_t st) {
+    const size_t N = (size_t)D * D * D;
+    uint32_t* a = (uint32_t*)work;
+    uint32_t* b = a + N;
+    const float R2 = R_cells * R_cells;  // float32(R*R), find_bubbles.cu:98
+    int omax = (int)std::ceil((double)R_cells);
+    while (omax > 0 && (float)(omax * omax) >= R2) --omax;  // largest o with o^2 < R^2
+    if (omax >= D) omax = periodic ? D / 2 : D - 1;
+    k_sphere_seed<<<grid_for(N), kThreads, 0, st>>>(a, sm, N, deltac, denom, 1.0 / zeta, sum, fc_mean);
+    CM21_LAUNCH_CHECK("k_sphere_seed");
+    k_sphere_pass<<<grid_for(N), kThreads, 0, st>>>(b, a, D, 2, omax, periodic);
+    CM21_LAUNCH_CHECK("k_sphere_pass");
+    k_sphere_pass<<<grid_for(N), kThreads, 0, st>>>(a, b, D, 1, omax, periodic);
+    CM21_LAUNCH_CHECK("k_sphere_pass");
+    k_sphere_pass<<<grid_for(N), kThreads, 0, st>>>(b, a, D, 0, omax, periodic);
+    CM21_LAUNCH_CHECK("k_sphere_pass");
+    k_sphere_apply<<<grid_for(N), kThreads, 0, st>>>(ion, b, N, R2);
+    CM21_LAUNCH_CHECK("k_sphere_apply");
+    return CM21_OK;
+}
+
+bool valid_dim(int D) { return D >= 2 && D <= 2048 && D % 2 == 0; }
+
+}  // namespace
+
+extern "C" int cm21_hii_filter(void* spec_out, const void* spec_in, int D, float delta_k, int filter_type,
+                               int ref_fallthrough, int korder, float R, float scale, void* stream) {
+    CM21_REQUIRE(spec_out && spec_in, "cm21_hii_filter: NULL spectrum");
+    CM21_REQUIRE(valid_dim(D), "cm21_hii_filter: D=%d must be even in [2,2048]", D);
+    CM21_REQUIRE(filter_type >= 0 && filter_type <= 2, "cm21_hii_filter: filter_type=%d not in {0,1,2}", filter_type);
+    CM21_REQUIRE(korder == 0 || korder == 1, "cm21_hii_filter: korder=%d not in {0,1}", korder);
+    WindowParams wp{delta_k, R, filter_type, ref_fallthrough ? 1 : 0, korder};
+    return launch_filter(spec_out, spec_in, D, wp, scale, (hipStream_t)stream);
+}
+
+extern "C" size_t cm21_reduce_workspace_bytes(void) { return align_up(sizeof(double) * kReduceBlocks, 256); }
+
+extern "C" int cm21_fcoll_sum(const float* smoothed, int D, float deltac, float denom, double* dev_sum,
+                              void* work, void* stream) {
+    CM21_REQUIRE(smoothed && dev_sum && work, "cm21_fcoll_sum: NULL argument");
+    CM21_REQUIRE(valid_dim(D), "cm21_fcoll_sum: D=%d must be even in [2,2048]", D);
+    return launch_fcoll_sum(smoothed, D, deltac, denom, dev_sum, work, (hipStream_t)stream);
+}
+
+extern "C" int cm21_fcoll_threshold(float* ionized, const float* smoothed, int D, float deltac, float denom,
+                                    double zeta, const double* dev_sum, float fc_mean, int final_step,
+                                    void* stream) {
+    CM21_REQUIRE(ionized && smoothed, "cm21_fcoll_threshold: NULL argument");
+    CM21_REQUIRE(valid_dim(D), "cm21_fcoll_threshold: D=%d must be even in [2,2048]", D);
+    CM21_REQUIRE(zeta > 0.0, "cm21_fcoll_threshold: zeta must be positive");
+    return launch_threshold(ionized, smoothed, D, deltac, denom, zeta, dev_sum, fc_mean, final_step,
+                            (hipStream_t)stream);
+}
+
+extern "C" size_t cm21_sphere_workspace_bytes(int D) {
+    return align_up(sizeof(uint32_t) * 2 * (size_t)D * D * D, 256);
+}
+
+extern "C" int cm21_update_sphere(float* ionized, const float* smoothed, int D, float deltac, float denom,
+                                  double zeta, const double* dev_sum, float fc_mean, float R_cells, int periodic,
+                                  void* work, void* stream) {
+    CM21_REQUIRE(ionized && smoothed && work, "cm21_update_sphere: NULL argument");
+    CM21_REQUIRE(valid_dim(D), "cm21_update_sphere: D=%d must be even in [2,2048]", D);
+    CM21_REQUIRE(zeta > 0.0, "cm21_update_sphere: zeta must be positive");
+    CM21_REQUIRE(R_cells >= 0.0f && R_cells <= 1024.0f, "cm21_update_sphere: R_cells=%g not in [0,1024]",
+                 (double)R_cells);
+    return launch_sphere(ionized, smoothed, D, deltac, denom, zeta, dev_sum, fc_mean, R_cells, periodic ? 1 : 0,
+                         work, (hipStream_t)stream);
+}
+
+// Workspace carve-up of the loop driver.
+namespace {
+struct LoopWork {
+    float2* spec;
+    float2* filt;
+    float* smoothed;
+    double* partial;
+    double* sums;
+    void* sphere;
+    size_t bytes;
+};
+
+LoopWork carve(void* base, int D, int n_radii, int update) {
+    const size_t N = (size_t)D * D * D;
+    const size_t Nc = (size_t)D * D * (D / 2 + 1);
+    char* p = (char*)base;
+    LoopWork w{};
+    size_t off = 0;
+    w.spec = (float2*)(p + off);
+    off = align_up(off + sizeof(float2) * Nc, 256);
+    w.filt = (float2*)(p + off);
+    off = align_up(off + sizeof(float2) * Nc, 256);
+    w.smoothed = (float*)(p + off);
+    off = align_up(off + sizeof(float) * N, 256);
+    w.partial = (double*)(p + off);
+    off = align_up(off + cm21_reduce_workspace_bytes(), 256);
+    w.sums = (double*)(p + off);
+    off = align_up(off + sizeof(double) * (size_t)(n_radii > 0 ? n_radii : 1), 256);
+    w.sphere = update ? (void*)(p + off) : nullptr;
+    if (update) off = align_up(off + cm21_sphere_workspace_bytes(D), 256);
+    w.bytes = off;
+    return w;
+}
+}  // namespace
+
+extern "C" size_t cm21_find_bubbles_workspace_bytes(int D, int n_radii, int update) {
+    if (!valid_dim(D)) return 0;
+    return carve(nullptr, D, n_radii, update).bytes;
+}
+
+extern "C" int cm21_find_bubbles(cm21_plan* plan, const float* delta, float* ionized, const float* radii,
+                                 const float* denoms, int n_radii, const cm21_bubble_params* prm, void* work,
+                                 size_t work_bytes, void* stream) {
+    CM21_REQUIRE(plan && delta && ionized && radii && denoms && prm && work, "cm21_find_bubbles: NULL argument");
+    const int D = plan_dim(plan);
+    CM21_REQUIRE(plan_batch(plan) == 1, "cm21_find_bubbles: plan must have batch 1");
+    CM21_REQUIRE(n_radii >= 1 && n_radii <= 4096, "cm21_find_bubbles: n_radii=%d out of range", n_radii);
+    CM21_REQUIRE(prm->filter_type >= 0 && prm->filter_type <= 2, "cm21_find_bubbles: filter_type=%d",
+                 prm->filter_type);
+    CM21_REQUIRE(prm->zeta > 0.0, "cm21_find_bubbles: zeta must be positive");
+    CM21_REQUIRE(prm->box_len > 0.0f, "cm21_find_bubbles: box_len must be positive");
+    for (int r = 0; r < n_radii; ++r) {
+        if (!(denoms[r] > 0.0f) || !std::isfinite(denoms[r]))
+            return cm21::fail(CM21_ERR_DOMAIN,
+                              "cm21_find_bubbles: denominator %g at R=%g is not positive and finite "
+                              "(sigma_min^2 <= S(R))",
+                              (double)denoms[r], (double)radii[r]);
+        if (prm->update && !(radii[r] * prm->sphere_r_scale <= 1024.0f))
+            return cm21::fail(CM21_ERR_INVALID, "cm21_find_bubbles: sphere radius %g cells exceeds 1024",
+                              (double)(radii[r] * prm->sphere_r_scale));
+    }
+    LoopWork w = carve(work, D, n_radii, prm->update);
+    if (work_bytes < w.bytes)
+        return cm21::fail(CM21_ERR_NOMEM, "cm21_find_bubbles: workspace %zu bytes < required %zu", work_bytes,
+                          w.bytes);
+    hipStream_t st = (hipStream_t)stream;
+    const size_t N = (size_t)D * D * D;
+
+    k_fill<<<grid_for(N), kThreads, 0, st>>>(ionized, N, 0.0f);
+    CM21_LAUNCH_CHECK("k_fill");
+    CM21_TRY(cm21_r2c(plan, delta, w.spec, stream));
+    const float dk = (float)(2.0 * M_PI / (double)prm->box_len);
+    const float scale = prm->linear_input ? (float)(1.0 / (double)N) : (float)((double)prm->growth_inv / (double)N);
+    for (int r = 0; r < n_radii; ++r) {
+        const bool final_step = r == n_radii - 1;
+        WindowParams wp{dk, radii[r], prm->filter_type, prm->ref_fallthrough ? 1 : 0, 0};
+        CM21_TRY(launch_filter(w.filt, w.spec, D, wp, scale, st));
+        CM21_TRY(cm21_c2r(plan, w.filt, w.smoothed, stream));
+        const double* sum = nullptr;
+        if (!prm->linear_input) {
+            CM21_TRY(launch_fcoll_sum(w.smoothed, D, prm->deltac, denoms[r], w.sums + r, w.partial, st));
+            sum = w.sums + r;
+        }
+        if (prm->update && !final_step) {
+            CM21_TRY(launch_sphere(ionized, w.smoothed, D, prm->deltac, denoms[r], prm->zeta, sum, prm->fc_mean,
+                                   radii[r] * prm->sphere_r_scale, prm->sphere_periodic, w.sphere, st));
+        } else {
+            CM21_TRY(launch_threshold(ionized, w.smoothed, D, prm->deltac, denoms[r], prm->zeta, sum, prm->fc_mean,
+                                      final_step ? 1 : 0, st));
+        }
+    }
+    return CM21_OK;
+}

@@ -1,0 +1,143 @@
+// Error reporting, ABI version and the rocFFT plans behind the C ABI.
+//
+// Plans replace pyfft.cuda.Plan (Programs/find_HII_bubbles.py:91): one plan
+// object holds a real-forward and a real-inverse rocFFT plan for `batch`
+// packed D^3 boxes, a preallocated work buffer and an execution-info object
+// whose stream is rebound on every call.
+#include <cstdarg>
+#include <cstdio>
+#include <mutex>
+#include <string>
+
+#include <rocfft/rocfft.h>
+
+#include "cm21_internal.hpp"
+
+namespace {
+thread_local std::string g_last_error;
+std::once_flag g_rocfft_once;
+rocfft_status g_rocfft_setup_status = rocfft_status_success;
+}  // namespace
+
+namespace cm21 {
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+void clear_error() { g_last_error.clear(); }
+}  // namespace cm21
+
+struct cm21_plan {
+    int D = 0;
+    int batch = 0;
+    rocfft_plan fwd = nullptr;
+    rocfft_plan inv = nullptr;
+    rocfft_execution_info info = nullptr;
+    void* work = nullptr;
+    size_t work_bytes = 0;
+};
+
+extern "C" int cm21_abi_version(void) { return CM21_ABI_VERSION; }
+
+extern "C" const char* cm21_last_error(void) { return g_last_error.c_str(); }
+
+#define CM21_FFT_CHECK(call)                                                        \
+    do {                                                                            \
+        rocfft_status st_ = (call);                                                 \
+        if (st_ != rocfft_status_success)                                           \
+            return cm21::fail(CM21_ERR_FFT, "%s failed (rocfft status %d)", #call,  \
+                              (int)st_);                                            \
+    } while (0)
+
+static void plan_release(cm21_plan* p) {
+    if (!p) return;
+    if (p->fwd) rocfft_plan_destroy(p->fwd);
+    if (p->inv) rocfft_plan_destroy(p->inv);
+    if (p->info) rocfft_execution_info_destroy(p->info);
+    if (p->work) (void)hipFree(p->work);
+    delete p;
+}
+
+extern "C" int cm21_plan_create(cm21_plan** out, int D, int batch) {
+    CM21_REQUIRE(out != nullptr, "cm21_plan_create: out is NULL");
+    *out = nullptr;
+    CM21_REQUIRE(D >= 2 && D <= 2048 && (D % 2) == 0, "cm21_plan_create: D=%d must be even in [2,2048]", D);
+    CM21_REQUIRE(batch >= 1 && batch <= 4096, "cm21_plan_create: batch=%d out of range", batch);
+    std::call_once(g_rocfft_once, [] { g_rocfft_setup_status = rocfft_setup(); });
+    if (g_rocfft_setup_status != rocfft_status_success)
+        return cm21::fail(CM21_ERR_FFT, "rocfft_setup failed (status %d)", (int)g_rocfft_setup_status);
+
+    cm21_plan* p = new cm21_plan();
+    p->D = D;
+    p->batch = batch;
+    const size_t len[3] = {(size_t)D, (size_t)D, (size_t)D};
+    rocfft_status st = rocfft_plan_create(&p->fwd, rocfft_placement_notinplace,
+                                          rocfft_transform_type_real_forward, rocfft_precision_single,
+                                          3, len, (size_t)batch, nullptr);
+    if (st == rocfft_status_success)
+        st = rocfft_plan_create(&p->inv, rocfft_placement_notinplace,
+                                rocfft_transform_type_real_inverse, rocfft_precision_single, 3, len,
+                                (size_t)batch, nullptr);
+    if (st != rocfft_status_success) {
+        plan_release(p);
+        return cm21::fail(CM21_ERR_FFT, "rocfft_plan_create(D=%d, batch=%d) failed (status %d)", D, batch, (int)st);
+    }
+    size_t w1 = 0, w2 = 0;
+    rocfft_plan_get_work_buffer_size(p->fwd, &w1);
+    rocfft_plan_get_work_buffer_size(p->inv, &w2);
+    p->work_bytes = w1 > w2 ? w1 : w2;
+    st = rocfft_execution_info_create(&p->info);
+    if (st != rocfft_status_success) {
+        plan_release(p);
+        return cm21::fail(CM21_ERR_FFT, "rocfft_execution_info_create failed (status %d)", (int)st);
+    }
+    if (p->work_bytes) {
+        hipError_t e = hipMalloc(&p->work, p->work_bytes);
+        if (e != hipSuccess) {
+            plan_release(p);
+            return cm21::fail(CM21_ERR_NOMEM, "hipMalloc of %zu-byte FFT work buffer failed: %s", p->work_bytes,
+                              hipGetErrorString(e));
+        }
+        st = rocfft_execution_info_set_work_buffer(p->info, p->work, p->work_bytes);
+        if (st != rocfft_status_success) {
+            plan_release(p);
+            return cm21::fail(CM21_ERR_FFT, "rocfft_execution_info_set_work_buffer failed (status %d)", (int)st);
+        }
+    }
+    *out = p;
+    return CM21_OK;
+}
+
+extern "C" int cm21_plan_destroy(cm21_plan* plan) {
+    plan_release(plan);
+    return CM21_OK;
+}
+
+extern "C" int cm21_r2c(cm21_plan* plan, const float* in, void* spec_out, void* stream) {
+    CM21_REQUIRE(plan && in && spec_out, "cm21_r2c: NULL argument");
+    CM21_FFT_CHECK(rocfft_execution_info_set_stream(plan->info, stream));
+    void* ib[1] = {const_cast<float*>(in)};
+    void* ob[1] = {spec_out};
+    CM21_FFT_CHECK(rocfft_execute(plan->fwd, ib, ob, plan->info));
+    return CM21_OK;
+}
+
+extern "C" int cm21_c2r(cm21_plan* plan, void* spec_in, float* out, void* stream) {
+    CM21_REQUIRE(plan && spec_in && out, "cm21_c2r: NULL argument");
+    CM21_FFT_CHECK(rocfft_execution_info_set_stream(plan->info, stream));
+    void* ib[1] = {spec_in};
+    void* ob[1] = {out};
+    CM21_FFT_CHECK(rocfft_execute(plan->inv, ib, ob, plan->info));
+    return CM21_OK;
+}
+
+// Plan dimension accessor used by the loop driver in other translation units.
+namespace cm21 {
+int plan_dim(const cm21_plan* p) { return p ? p->D : 0; }
+int plan_batch(const cm21_plan* p) { return p ? p->batch : 0; }
+}  // namespace cm21

@@ -1,0 +1,184 @@
+// Initial conditions on the GPU: the Gaussian random field written directly
+// as a Hermitian r2c half spectrum, the nearest-pick subsample and the
+// Zel'dovich displacement ("velocity") spectra.
+//
+// Reference: Programs/initialize.cu:58-95 (init_kernel), :133-146 (subsample),
+// :148-190 (set_velocity); Programs/initialize.py:33-111 (driver).
+#include <cmath>
+
+#include "cm21_internal.hpp"
+#include "cm21_window.hpp"
+
+using namespace cm21;
+
+namespace {
+
+// Philox4x32-10 (Salmon et al., SC'11): ten rounds of two 32x32->64-bit
+// multiplies; the key is bumped by the Weyl constants before rounds 2..10.
+__device__ inline uint4 philox4x32_10(uint4 c, uint2 k) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k.x += 0x9E3779B9u;
+            k.y += 0xBB67AE85u;
+        }
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+    }
+    return c;
+}
+
+// ((u >> 8) + 0.5) * 2^-24: an exact value in (0, 1).
+__device__ inline double uniform_open01(uint32_t u) { return ((double)(u >> 8) + 0.5) * (1.0 / 16777216.0); }
+
+// FP64 Box-Muller: r = sqrt(-2 ln a), theta = 2 pi b.
+__device__ inline void box_muller(uint32_t u0, uint32_t u1, double& g1, double& g2) {
+    const double a = uniform_open01(u0), b = uniform_open01(u1);
+    const double r = sqrt(-2.0 * log(a));
+    const double th = (2.0 * M_PI) * b;
+    g1 = r * cos(th);
+    g2 = r * sin(th);
+}
+
+struct GrfParams {
+    double dk;       // 2 pi / box_len
+    double amp;      // P(k) normalisation
+    double n_index;  // spectral slope
+    double k_cut;    // Gaussian cut-off scale
+    double volume;   // box_len^3
+    uint32_t seed, box_id;
+};
+
+__global__ void k_grf(float2* __restrict__ spec, int D, size_t Nc, GrfParams gp) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= Nc) return;
+    const int H = D / 2 + 1;
+    const int c = (int)(idx % H);
+    const size_t ab = idx / H;
+    const int b = (int)(ab % D);
+    const int a = (int)(ab / D);
+    if (a == 0 && b == 0 && c == 0) {
+        spec[idx] = make_float2(0.0f, 0.0f);
+        return;
+    }
+    // Modes on the c = 0 and c = D/2 planes pair with (-a, -b) in the same
+    // plane; the pair shares the random numbers of its lower linear index.
+    const bool edge = (c == 0) || (c == D / 2);
+    const int ma = (D - a) % D, mb = (D - b) % D;
+    const long lin_self = (long)a * D + b, lin_mirror = (long)ma * D + mb;
+    const bool use_mirror = edge && lin_mirror < lin_self;
+    const bool selfconj = edge && ma == a && mb == b;
+    const long ra = use_mirror ? ma : a, rb = use_mirror ? mb : b;
+    const unsigned long long rep = ((unsigned long long)(ra * D + rb)) * (unsigned long long)H + (unsigned long long)c;
+    const uint4 u = philox4x32_10(make_uint4((uint32_t)rep, (uint32_t)(rep >> 32), 0u, 0u), make_uint2(gp.seed, gp.box_id));
+    double g1, g2;
+    box_muller(u.x, u.y, g1, g2);
+
+    const double na = mode_of(a, D), nb = mode_of(b, D), nc = c;
+    const double m2 = na * na + nb * nb + nc * nc;
+    const double k = gp.dk * sqrt(m2);
+    const double x = k / gp.k_cut;
+    const double P = gp.amp * (pow(k, gp.n_index) * exp(-(x * x)));
+    const double amp = sqrt(P / gp.volume);
+    double re, im;
+    if (selfconj) {
+        re = amp * g1;
+        im = 0.0;
+    } else {
+        re = amp * g1 / 1.4142135623730951;
+        im = amp * g2 / 1.4142135623730951;
+        if (use_mirror) im = -im;
+    }
+    spec[idx] = make_float2((float)re, (float)im);
+}
+
+__global__ void k_subsample(float* __restrict__ small, const float* __restrict__ large, int D, int sw, float pf) {
+    const size_t n = (size_t)sw * sw * sw;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int k = (int)(p % sw);
+    const int j = (int)((p / sw) % sw);
+    const int i = (int)(p / ((size_t)sw * sw));
+    // floor(k*pixel_factor + 0.5): product in float, + 0.5 in double.
+    const int lk = (int)floor((double)__fmul_rn((float)k, pf) + 0.5);
+    const int lj = (int)floor((double)__fmul_rn((float)j, pf) + 0.5);
+    const int li = (int)floor((double)__fmul_rn((float)i, pf) + 0.5);
+    if (li >= D || lj >= D || lk >= D) return;  // never taken for pf = D/sw
+    small[p] = large[((size_t)li * D + lj) * D + lk];
+}
+
+__global__ void k_velocity(float2* __restrict__ out, const float2* __restrict__ in, int D, size_t Nc, float dk,
+                           int comp, int filter_type, float R) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= Nc) return;
+    const int H = D / 2 + 1;
+    const int c = (int)(idx % H);
+    const size_t ab = idx / H;
+    const int b = (int)(ab % D);
+    const int a = (int)(ab / D);
+    const float k0 = __fmul_rn((float)mode_of(a, D), dk);
+    const float k1 = __fmul_rn((float)mode_of(b, D), dk);
+    const float k2 = __fmul_rn((float)c, dk);
+    // initialize.cu:162: k_sq = k_x^2 + k_y^2 + k_z^2 with x the slowest axis.
+    const float ksq = __fadd_rn(__fadd_rn(__fmul_rn(k0, k0), __fmul_rn(k1, k1)), __fmul_rn(k2, k2));
+    const int idx_c = comp == 0 ? a : (comp == 1 ? b : c);
+    const float kc = comp == 0 ? k0 : (comp == 1 ? k1 : k2);
+    if (ksq == 0.0f || idx_c == D / 2) {
+        out[idx] = make_float2(0.0f, 0.0f);
+        return;
+    }
+    const float f = __fdiv_rn(kc, ksq);
+    const float2 v = in[idx];
+    float2 r = make_float2(-__fmul_rn(f, v.y), __fmul_rn(f, v.x));  // (i f) * v
+    if (filter_type >= 0) {
+        WindowParams wp{dk, R, filter_type, 1, 1};
+        float w0, w2;
+        bool cut;
+        window_of(__fsqrt_rn(ksq), wp, w0, cut, w2);
+        r = apply_window(r, w0, cut, w2, 1.0f);
+    }
+    out[idx] = r;
+}
+
+bool valid_dim(int D) { return D >= 2 && D <= 2048 && D % 2 == 0; }
+
+}  // namespace
+
+extern "C" int cm21_grf_init(void* spec_out, int D, double box_len, double amp, double n_index, double k_cut,
+                             uint32_t seed, uint32_t box_id, void* stream) {
+    CM21_REQUIRE(spec_out, "cm21_grf_init: NULL spectrum");
+    CM21_REQUIRE(valid_dim(D), "cm21_grf_init: D=%d must be even in [2,2048]", D);
+    CM21_REQUIRE(box_len > 0.0 && k_cut > 0.0 && amp >= 0.0, "cm21_grf_init: box_len, k_cut must be > 0, amp >= 0");
+    const size_t Nc = (size_t)D * D * (D / 2 + 1);
+    GrfParams gp{2.0 * M_PI / box_len, amp, n_index, k_cut, box_len * box_len * box_len, seed, box_id};
+    k_grf<<<grid_for(Nc), kThreads, 0, (hipStream_t)stream>>>((float2*)spec_out, D, Nc, gp);
+    CM21_LAUNCH_CHECK("k_grf");
+    return CM21_OK;
+}
+
+extern "C" int cm21_subsample(float* small, const float* large, int D, int sw, float pixel_factor, void* stream) {
+    CM21_REQUIRE(small && large, "cm21_subsample: NULL argument");
+    CM21_REQUIRE(D >= 1 && sw >= 1 && sw <= D && D <= 4096, "cm21_subsample: D=%d sw=%d", D, sw);
+    // The largest pick is floor((sw-1)*pf + 0.5); it must stay inside the large box.
+    const double last = std::floor((double)((float)(sw - 1) * pixel_factor) + 0.5);
+    CM21_REQUIRE(pixel_factor > 0.0f && last < (double)D, "cm21_subsample: pixel_factor=%g reads past D=%d",
+                 (double)pixel_factor, D);
+    const size_t n = (size_t)sw * sw * sw;
+    k_subsample<<<grid_for(n), kThreads, 0, (hipStream_t)stream>>>(small, large, D, sw, pixel_factor);
+    CM21_LAUNCH_CHECK("k_subsample");
+    return CM21_OK;
+}
+
+extern "C" int cm21_velocity(void* spec_out, const void* spec_in, int D, float delta_k, int comp, int filter_type,
+                             float R, void* stream) {
+    CM21_REQUIRE(spec_out && spec_in, "cm21_velocity: NULL spectrum");
+    CM21_REQUIRE(valid_dim(D), "cm21_velocity: D=%d must be even in [2,2048]", D);
+    CM21_REQUIRE(comp >= 0 && comp <= 2, "cm21_velocity: comp=%d not in {0,1,2}", comp);
+    CM21_REQUIRE(filter_type <= 2, "cm21_velocity: filter_type=%d", filter_type);
+    const size_t Nc = (size_t)D * D * (D / 2 + 1);
+    k_velocity<<<grid_for(Nc), kThreads, 0, (hipStream_t)stream>>>((float2*)spec_out, (const float2*)spec_in, D, Nc,
+                                                                   delta_k, comp, filter_type, R);
+    CM21_LAUNCH_CHECK("k_velocity");
+    return CM21_OK;
+}

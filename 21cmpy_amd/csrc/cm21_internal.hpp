@@ -1,0 +1,86 @@
+// Internal helpers shared by the cm21 HIP sources: error reporting across the
+// C ABI, launch geometry and deterministic wavefront/block reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "cm21.h"
+
+namespace cm21 {
+
+// Record a formatted message for cm21_last_error() and return `code`.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+constexpr int kThreads = 256;          // threads per block for streaming kernels
+constexpr int kWaves = kThreads / 64;  // wavefronts per block (wave64 on CDNA)
+constexpr int kReduceBlocks = 1024;    // fixed grid for deterministic reductions
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+inline unsigned grid_for(size_t n, int threads = kThreads) {
+    return (unsigned)((n + threads - 1) / threads);
+}
+
+// Sum over the 64 lanes of a wavefront; the tree order is fixed, so the
+// result is bitwise reproducible.  Lane 0 holds the total.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Block-wide FP64 sum in a fixed order; thread 0 receives the total.
+__device__ inline double block_sum(double v) {
+    __shared__ double part[kWaves];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) part[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < kWaves; ++i) r += part[i];
+    }
+    return r;
+}
+
+// Signed mode number of FFT index i on an axis of length D; index D/2 maps
+// to +D/2 (Programs/find_bubbles.cu:22-24).
+__device__ inline int mode_of(int i, int D) { return (i > D / 2) ? i - D : i; }
+
+}  // namespace cm21
+
+#define CM21_HIP_CHECK(call)                                                        \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return cm21::fail(CM21_ERR_HIP, "%s failed at %s:%d: %s", #call,        \
+                              __FILE__, __LINE__, hipGetErrorString(e_));           \
+    } while (0)
+
+#define CM21_LAUNCH_CHECK(name)                                                     \
+    do {                                                                            \
+        hipError_t e_ = hipGetLastError();                                          \
+        if (e_ != hipSuccess)                                                       \
+            return cm21::fail(CM21_ERR_HIP, "launch of %s failed: %s", name,       \
+                              hipGetErrorString(e_));                               \
+    } while (0)
+
+#define CM21_REQUIRE(cond, ...)                                                     \
+    do {                                                                            \
+        if (!(cond)) return cm21::fail(CM21_ERR_INVALID, __VA_ARGS__);              \
+    } while (0)
+
+#define CM21_TRY(call)                                                              \
+    do {                                                                            \
+        int s_ = (call);                                                            \
+        if (s_ != CM21_OK) return s_;                                               \
+    } while (0)

@@ -1,0 +1,194 @@
+/*
+ * cm21.h - C ABI of the MI355X (gfx950) semi-numerical reionization hot path.
+ *
+ * Every entry point is asynchronous on the HIP stream it is given (a
+ * hipStream_t passed as void*, NULL = the null stream), takes plain device
+ * pointers owned by the caller, and returns an int status: CM21_OK (0) or a
+ * negative CM21_ERR_* code with a message in cm21_last_error() (thread-local).
+ * No exception or abort crosses this boundary.  The library owns only opaque
+ * plan handles (rocFFT plans and their work buffers).
+ *
+ * Layouts.  A real box is float32 (D,D,D), C order, axis 2 fastest.  A
+ * spectrum is the r2c half spectrum: float2 (re,im) interleaved, (D,D,D/2+1),
+ * axis 2 fastest.  Spectra are "c2r-ready": the unnormalised inverse real
+ * transform (cm21_c2r) of a spectrum gives the real-space field.  cm21_r2c is
+ * unnormalised (a forward transform of a field gives N times its c2r-ready
+ * spectrum).
+ *
+ * Each declaration names the reference interface it replaces (file:line in
+ * yunfanz/21cmpy).  Host-side scalars that the reference computes in Python
+ * (the radius list, S(R), denominators, the spectrum normalisation) are
+ * passed in by the caller.
+ */
+#ifndef CM21_H
+#define CM21_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CM21_ABI_VERSION 1
+
+#define CM21_OK 0
+#define CM21_ERR_INVALID (-1) /* bad argument (sizes, null pointers, flags) */
+#define CM21_ERR_HIP (-2)     /* HIP runtime or kernel launch error */
+#define CM21_ERR_FFT (-3)     /* rocFFT error */
+#define CM21_ERR_DOMAIN (-4)  /* physics domain error, e.g. sigma_min^2 <= S(R) */
+#define CM21_ERR_NOMEM (-5)   /* workspace too small or allocation failed */
+
+typedef struct cm21_plan cm21_plan;
+
+/* Version and last error (no reference counterpart: PyCUDA raised Python
+ * exceptions, Programs/find_HII_bubbles.py:116-118). */
+int cm21_abi_version(void);
+const char* cm21_last_error(void);
+
+/* ------------------------------------------------------------------ FFT
+ * Replaces pyfft.cuda.Plan(shape, complex64) and Plan.execute(buf,
+ * inverse=...) (Programs/find_HII_bubbles.py:91,144,150; initialize.py:79-80;
+ * delta_T.py:84-86).  The reference runs complex-to-complex transforms of real
+ * data; these are the equivalent real transforms on the half spectrum.
+ * A plan serves `batch` boxes of side D laid out back to back.  cm21_c2r may
+ * overwrite its input spectrum. */
+int cm21_plan_create(cm21_plan** out, int D, int batch);
+int cm21_plan_destroy(cm21_plan* plan);
+int cm21_r2c(cm21_plan* plan, const float* in, void* spec_out, void* stream);
+int cm21_c2r(cm21_plan* plan, void* spec_in, float* out, void* stream);
+
+/* ------------------------------------------------------------------ filter
+ * Replaces HII_filter (Programs/find_bubbles.cu:12-42, initialize.cu:10-40).
+ * spec_out = window(|k| R) * spec_in * scale on the half spectrum (in place
+ * allowed).  filter_type 0/1/2 as INIT_PARAMS.HII_FILTER; ref_fallthrough=1
+ * reproduces the reference switch without breaks (0 -> top-hat * k-cut *
+ * Gaussian(0.2659 kR), 1 -> k-cut * Gaussian(0.2659 kR), 2 -> Gaussian(0.643
+ * kR)); 0 gives the textbook windows.  korder selects the order of the |k|^2
+ * sum: 0 = fastest axis first (find_bubbles.cu:26), 1 = slowest first
+ * (initialize.cu:24).  delta_k = 2 pi / box_len in float32. */
+int cm21_hii_filter(void* spec_out, const void* spec_in, int D, float delta_k,
+                    int filter_type, int ref_fallthrough, int korder, float R,
+                    float scale, void* stream);
+
+/* ------------------------------------------------------------------ f_coll
+ * Replaces fcoll_kernel + gpuarray.sum (Programs/find_bubbles.cu:43-58;
+ * find_HII_bubbles.py:156-159): dev_sum[0] = sum over the box of
+ * erfcf((deltac - smoothed)/denom), accumulated in FP64 with a fixed
+ * reduction order (deterministic).  `work` holds at least
+ * cm21_reduce_workspace_bytes() bytes. */
+size_t cm21_reduce_workspace_bytes(void);
+int cm21_fcoll_sum(const float* smoothed, int D, float deltac, float denom,
+                   double* dev_sum, void* work, void* stream);
+
+/* Replaces fcoll_kernel + update_kernel (final_step = 0) or final_kernel
+ * (final_step = 1) (Programs/find_bubbles.cu:43-71,142-158;
+ * find_HII_bubbles.py:155-175).  f = erfcf((deltac - smoothed)/denom),
+ * optionally normalised by float32(fc_mean / (dev_sum[0]/N)) when dev_sum is
+ * not NULL (nonlinear input); cells with ionized == 1 are skipped, cells with
+ * f >= 1/zeta (compared in double) become 1, and on the final step the rest
+ * become f*zeta. */
+int cm21_fcoll_threshold(float* ionized, const float* smoothed, int D,
+                         float deltac, float denom, double zeta,
+                         const double* dev_sum, float fc_mean, int final_step,
+                         void* stream);
+
+/* Replaces update_sphere_kernel (Programs/find_bubbles.cu:73-140): every cell
+ * whose integer squared distance to a cell meeting the threshold is below
+ * float32(R_cells^2) becomes 1.  periodic = 0 clips at the box faces as the
+ * reference does; 1 wraps.  Computed as three separable min-plus passes
+ * (O(N R)) instead of the reference's O(R^3) loop per ionizing cell.  `work`
+ * holds at least cm21_sphere_workspace_bytes(D) bytes.  R_cells <= 1024. */
+size_t cm21_sphere_workspace_bytes(int D);
+int cm21_update_sphere(float* ionized, const float* smoothed, int D,
+                       float deltac, float denom, double zeta,
+                       const double* dev_sum, float fc_mean, float R_cells,
+                       int periodic, void* work, void* stream);
+
+/* ------------------------------------------------------------------ loop
+ * Replaces conv_bubbles (Programs/find_HII_bubbles.py:17-193): one forward
+ * transform of delta, then for each radius the filter, inverse transform,
+ * optional FP64 box mean and threshold/paint, all on `stream` with no host
+ * synchronisation.  radii/denoms are host arrays of n_radii float32 values
+ * (find_HII_bubbles.py:121-128); the last radius is the final step.
+ * `ionized` receives the ionized-fraction box (1 - xH).  `work` holds at
+ * least cm21_find_bubbles_workspace_bytes(D, n_radii, update) bytes. */
+typedef struct cm21_bubble_params {
+    float box_len;        /* Mpc; DELTA_K = float32(2 pi / box_len) */
+    float deltac;         /* barrier DELTAC (find_HII_bubbles.py:57,66) */
+    double zeta;          /* ionizing efficiency ZETA */
+    float growth_inv;     /* fgrowth = deltac/1.686, applied when !linear_input */
+    float fc_mean;        /* fc_mean_ps (find_HII_bubbles.py:60) */
+    int linear_input;     /* LE: 1 = linear z=0 input, no normalisation */
+    int filter_type;      /* HII_FILTER: 0, 1 or 2 */
+    int ref_fallthrough;  /* 1 = reference window chain */
+    int update;           /* 0 = centre pixel, 1 = sphere paint */
+    int sphere_periodic;  /* sphere paint wraps (1) or clips (0) */
+    float sphere_r_scale; /* R_cells = R * sphere_r_scale (1 = reference) */
+} cm21_bubble_params;
+
+size_t cm21_find_bubbles_workspace_bytes(int D, int n_radii, int update);
+int cm21_find_bubbles(cm21_plan* plan, const float* delta, float* ionized,
+                      const float* radii, const float* denoms, int n_radii,
+                      const cm21_bubble_params* params, void* work,
+                      size_t work_bytes, void* stream);
+
+/* ------------------------------------------------------------------ init
+ * Replaces init_kernel + MRG32k3a normals + complex assembly
+ * (Programs/initialize.cu:58-95; initialize.py:63-72): writes the c2r-ready
+ * Hermitian half spectrum of a Gaussian random field with
+ * P(k) = amp * k^n_index * exp(-(k/k_cut)^2), P(0) = 0, from Philox4x32-10
+ * keyed by (seed, box_id).  Paired modes get (g1 + i g2) sqrt(P/V)/sqrt(2),
+ * self-conjugate modes g1 sqrt(P/V), the DC mode 0. */
+int cm21_grf_init(void* spec_out, int D, double box_len, double amp,
+                  double n_index, double k_cut, uint32_t seed, uint32_t box_id,
+                  void* stream);
+
+/* Replaces subsample (Programs/initialize.cu:133-146): small[i,j,k] =
+ * large[l(i),l(j),l(k)], l(i) = floor(float32(i*pixel_factor) + 0.5). */
+int cm21_subsample(float* small, const float* large, int D, int sw,
+                   float pixel_factor, void* stream);
+
+/* Replaces set_velocity followed by HII_filter (Programs/initialize.cu:148-190,
+ * initialize.py:103-104): spec_out = i k_comp / k^2 * spec_in, zero at k = 0
+ * and on the Nyquist plane of axis comp, then the window of filter_type at R
+ * (filter_type < 0: no window).  comp = 0, 1, 2 is the array axis. */
+int cm21_velocity(void* spec_out, const void* spec_in, int D, float delta_k,
+                  int comp, int filter_type, float R, void* stream);
+
+/* ------------------------------------------------------------------ perturb
+ * Replaces move_mass + the renormalisation (Programs/perturb_field.cu:90-133;
+ * perturb_field.py:93-105): the DIM^3 particles at cell centres move by
+ * v*disp_scale (box units; v at the HII cell floor(i/pixel_factor)) on the
+ * sw^3 grid and deposit float(1 + growth_init*delta); scheme 0 = NGP
+ * (reference truncation), 1 = cloud-in-cell.  updated = sum/MASS_FACTOR - 1. */
+int cm21_move_mass(float* updated, const float* deltax, const float* vx,
+                   const float* vy, const float* vz, int DIM, int sw,
+                   float disp_scale, float growth_init, int scheme,
+                   void* stream);
+
+/* ------------------------------------------------------------------ delta_T
+ * Replaces the host NumPy brightness temperature (Programs/delta_T.py:73-76):
+ * dT = (tb_const*xH)*(1+delta) in float32; dev_sum[0] = FP64 sum of dT. */
+int cm21_delta_T(float* dT, const float* xH, const float* delta, int D,
+                 float tb_const, double* dev_sum, void* work, void* stream);
+
+/* Replaces pbox_kernel and its host driver (Programs/delta_T.cu:5-29,
+ * delta_T.py:80-98).  Transforms (field/mean - 1) V/N (times mean when
+ * dimensional) and bins k^3 |d_k|^2 / (2 pi^2 V) by bin_lut[|n|^2] (an int32
+ * table of 3(D/2)^2 + 1 entries: bin index, -1 or >= nbins to drop).  Writes
+ * FP64 sums ps_sum[nbins], k_sum[nbins] and exact counts[nbins] (uint64) on
+ * the device, deterministically.  1 <= nbins <= 512.  field_sum is the FP64 device sum of field.
+ * `work` holds at least cm21_pk_workspace_bytes(D, nbins) bytes. */
+size_t cm21_pk_workspace_bytes(int D, int nbins);
+int cm21_power_spectrum(cm21_plan* plan, const float* field,
+                        const double* field_sum, int D, double box_len,
+                        const int32_t* bin_lut, int nbins, int dimensional,
+                        double* ps_sum, double* k_sum, uint64_t* counts,
+                        void* work, size_t work_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CM21_H */
