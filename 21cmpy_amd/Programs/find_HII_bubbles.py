@@ -89,7 +89,7 @@ def bubble_setup(D, box_len, scale, deltac, sigma_min2, sig0_fn, py2_int_scale=F
 def conv_bubbles(deltax, param_dict, Z=None, scale=None, fil=1, update=0, LE=False, visualize=0, quiet=False,
                  *, deltac=None, growth=None, sigma_min2=None, zeta=None, box_len=None, sig0=None,
                  strict_filter=False, py2_int_scale=False, sphere_periodic=False, sphere_in_cells=False,
-                 stream=None):
+                 fused=True, stream=None):
     """Ionized box from a density box by the excursion-set radius loop.
 
     deltax: float32 (D,D,D) NumPy array or CUDA tensor (returned in the same
@@ -99,6 +99,9 @@ def conv_bubbles(deltax, param_dict, Z=None, scale=None, fil=1, update=0, LE=Fal
     1 sphere paint (R in Mpc used as a cell radius, clipped at the faces, as the
     reference; sphere_in_cells/sphere_periodic give R/cell and wrap-around).
     LE: True for linear z=0 input (no growth scaling, no normalisation).
+    fused: use the fused, pruned transform kernels where supported (centre-pixel
+    update, filter 1 or 2, D in 64/96/128/160/256/512); False forces rocFFT plus
+    separate filter and threshold kernels.
     """
     if visualize:
         raise ValueError("visualize is not supported by the GPU path")
@@ -127,7 +130,7 @@ def conv_bubbles(deltax, param_dict, Z=None, scale=None, fil=1, update=0, LE=Fal
         fc_mean=float(fc_mean), linear_input=1 if LE else 0, filter_type=int(fil),
         ref_fallthrough=0 if strict_filter else 1, update=int(update),
         sphere_periodic=1 if sphere_periodic else 0,
-        sphere_r_scale=(1.0 / cell) if sphere_in_cells else 1.0)
+        sphere_r_scale=(1.0 / cell) if sphere_in_cells else 1.0, fused=1 if fused else 0)
     is_tensor = isinstance(deltax, torch.Tensor)
     dev = torch.device("cuda")
     d = deltax if is_tensor else torch.from_numpy(np.ascontiguousarray(deltax, dtype=np.float32))

@@ -43,6 +43,7 @@ class BubbleParams(ctypes.Structure):
         ("update", ctypes.c_int),
         ("sphere_periodic", ctypes.c_int),
         ("sphere_r_scale", ctypes.c_float),
+        ("fused", ctypes.c_int),
     ]
 
 
@@ -72,7 +73,34 @@ SIGNATURES = {
     "cm21_delta_T": (I, [VP, VP, VP, I, F, VP, VP, VP]),
     "cm21_pk_workspace_bytes": (SZ, [I, I]),
     "cm21_power_spectrum": (I, [VP, VP, VP, I, D, VP, I, I, VP, VP, VP, VP, SZ, VP]),
+    "cm21_profile_enable": (I, [I]),
+    "cm21_profile_collect": (I, [c_double_p, ctypes.POINTER(ctypes.c_int64), I]),
 }
+
+STAGES = ["r2c", "filter", "c2r", "fcoll_sum", "threshold", "sphere", "fused_columns", "fused_rows",
+          "fused_lines"]
+
+
+def profile_enable(stages=True):
+    """stages: True (all), False/None (off) or an iterable of stage names."""
+    if stages is True:
+        mask = -1
+    elif not stages:
+        mask = 0
+    else:
+        mask = 0
+        for name in stages:
+            mask |= 1 << STAGES.index(name)
+    call("cm21_profile_enable", mask)
+
+
+def profile_collect():
+    """{stage: (total_ms, launches)} of the instrumented launches since the last collect."""
+    n = len(STAGES)
+    ms = (ctypes.c_double * n)()
+    cnt = (ctypes.c_int64 * n)()
+    call("cm21_profile_collect", ms, cnt, n)
+    return {name: (ms[i], cnt[i]) for i, name in enumerate(STAGES) if cnt[i] > 0}
 
 _lib = None
 

@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "lib21cm_hip.so")
 SOURCES = [
     "cm21_core.hip",
     "cm21_bubbles.hip",
-    "cm21_fft.hip",
+    "cm21_fused.hip",
     "cm21_init.hip",
     "cm21_perturb.hip",
     "cm21_deltat.hip",

@@ -17,7 +17,9 @@
 namespace cm21 {
 int plan_dim(const cm21_plan* p);
 int plan_batch(const cm21_plan* p);
+const float2* plan_twiddles(const cm21_plan* p);
 }
+#include "cm21_fused_api.hpp"
 
 using namespace cm21;
 
@@ -242,6 +244,9 @@ struct LoopWork {
     double* partial;
     double* sums;
     void* sphere;
+    unsigned long long* mask;
+    double* lpart;
+    unsigned int* ticket;
     size_t bytes;
 };
 
@@ -263,6 +268,12 @@ LoopWork carve(void* base, int D, int n_radii, int update) {
     off = align_up(off + sizeof(double) * (size_t)(n_radii > 0 ? n_radii : 1), 256);
     w.sphere = update ? (void*)(p + off) : nullptr;
     if (update) off = align_up(off + cm21_sphere_workspace_bytes(D), 256);
+    w.mask = (unsigned long long*)(p + off);
+    off = align_up(off + sizeof(unsigned long long) * ((N + 63) / 64), 256);
+    w.lpart = (double*)(p + off);
+    off = align_up(off + sizeof(double) * (fused_partials(D) + 1), 256);
+    w.ticket = (unsigned int*)(p + off);
+    off = align_up(off + sizeof(unsigned int), 256);
     w.bytes = off;
     return w;
 }
@@ -301,27 +312,63 @@ extern "C" int cm21_find_bubbles(cm21_plan* plan, const float* delta, float* ion
     hipStream_t st = (hipStream_t)stream;
     const size_t N = (size_t)D * D * D;
 
-    k_fill<<<grid_for(N), kThreads, 0, st>>>(ionized, N, 0.0f);
-    CM21_LAUNCH_CHECK("k_fill");
-    CM21_TRY(cm21_r2c(plan, delta, w.spec, stream));
     const float dk = (float)(2.0 * M_PI / (double)prm->box_len);
     const float scale = prm->linear_input ? (float)(1.0 / (double)N) : (float)((double)prm->growth_inv / (double)N);
+    {
+        ProfToken t = prof_begin(CM21_STAGE_R2C, st);
+        CM21_TRY(cm21_r2c(plan, delta, w.spec, stream));
+        prof_end(t, st);
+    }
+    const FusedShape* sh = fused_shape(D);
+    if (prm->fused && sh && prm->update == 0 && (prm->filter_type == 1 || prm->filter_type == 2)) {
+        const float2* tw = plan_twiddles(plan);
+        CM21_TRY(fused_mask_clear(w.mask, (N + 63) / 64, st));
+        CM21_TRY(fused_ticket_clear(w.ticket, st));
+        for (int r = 0; r < n_radii; ++r) {
+            const bool final_step = r == n_radii - 1;
+            FusedParams fp = fused_params(D, dk, radii[r], prm->filter_type, prm->ref_fallthrough, scale);
+            CM21_TRY(fused_radius(*sh, w.spec, w.filt, tw, fp, st));
+            const double* sum = nullptr;
+            if (!prm->linear_input) {
+                LineEpilogue es = line_epilogue(prm->deltac, denoms[r], prm->zeta, nullptr, prm->fc_mean, w.mask,
+                                                ionized, w.lpart, w.sums + r, w.ticket, (double)N);
+                CM21_TRY(fused_lines(*sh, 0, w.filt, tw, fp, es, st));
+                sum = w.sums + r;
+            }
+            LineEpilogue ep = line_epilogue(prm->deltac, denoms[r], prm->zeta, sum, prm->fc_mean, w.mask, ionized,
+                                            w.lpart, nullptr, w.ticket, (double)N);
+            CM21_TRY(fused_lines(*sh, final_step ? 2 : 1, w.filt, tw, fp, ep, st));
+        }
+        return CM21_OK;
+    }
+    k_fill<<<grid_for(N), kThreads, 0, st>>>(ionized, N, 0.0f);
+    CM21_LAUNCH_CHECK("k_fill");
     for (int r = 0; r < n_radii; ++r) {
         const bool final_step = r == n_radii - 1;
         WindowParams wp{dk, radii[r], prm->filter_type, prm->ref_fallthrough ? 1 : 0, 0};
+        ProfToken t = prof_begin(CM21_STAGE_FILTER, st);
         CM21_TRY(launch_filter(w.filt, w.spec, D, wp, scale, st));
+        prof_end(t, st);
+        t = prof_begin(CM21_STAGE_C2R, st);
         CM21_TRY(cm21_c2r(plan, w.filt, w.smoothed, stream));
+        prof_end(t, st);
         const double* sum = nullptr;
         if (!prm->linear_input) {
+            t = prof_begin(CM21_STAGE_FCOLL_SUM, st);
             CM21_TRY(launch_fcoll_sum(w.smoothed, D, prm->deltac, denoms[r], w.sums + r, w.partial, st));
+            prof_end(t, st);
             sum = w.sums + r;
         }
         if (prm->update && !final_step) {
+            t = prof_begin(CM21_STAGE_SPHERE, st);
             CM21_TRY(launch_sphere(ionized, w.smoothed, D, prm->deltac, denoms[r], prm->zeta, sum, prm->fc_mean,
                                    radii[r] * prm->sphere_r_scale, prm->sphere_periodic, w.sphere, st));
+            prof_end(t, st);
         } else {
+            t = prof_begin(CM21_STAGE_THRESHOLD, st);
             CM21_TRY(launch_threshold(ionized, w.smoothed, D, prm->deltac, denoms[r], prm->zeta, sum, prm->fc_mean,
                                       final_step ? 1 : 0, st));
+            prof_end(t, st);
         }
     }
     return CM21_OK;

@@ -8,9 +8,11 @@
 #include <cstdio>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include <rocfft/rocfft.h>
 
+#include "cm21_dft.hpp"
 #include "cm21_internal.hpp"
 
 namespace {
@@ -40,6 +42,7 @@ struct cm21_plan {
     rocfft_execution_info info = nullptr;
     void* work = nullptr;
     size_t work_bytes = 0;
+    float2* twiddles = nullptr;  // w_D^m = exp(+2 pi i m/D), m < D, for the fused transforms
 };
 
 extern "C" int cm21_abi_version(void) { return CM21_ABI_VERSION; }
@@ -60,6 +63,7 @@ static void plan_release(cm21_plan* p) {
     if (p->inv) rocfft_plan_destroy(p->inv);
     if (p->info) rocfft_execution_info_destroy(p->info);
     if (p->work) (void)hipFree(p->work);
+    if (p->twiddles) (void)hipFree(p->twiddles);
     delete p;
 }
 
@@ -109,6 +113,16 @@ extern "C" int cm21_plan_create(cm21_plan** out, int D, int batch) {
             return cm21::fail(CM21_ERR_FFT, "rocfft_execution_info_set_work_buffer failed (status %d)", (int)st);
         }
     }
+    {
+        std::vector<float2> tw((size_t)D);
+        for (int m = 0; m < D; ++m) tw[m] = cm21::twiddle(m, D);
+        hipError_t e = hipMalloc(&p->twiddles, sizeof(float2) * (size_t)D);
+        if (e == hipSuccess) e = hipMemcpy(p->twiddles, tw.data(), sizeof(float2) * (size_t)D, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            plan_release(p);
+            return cm21::fail(CM21_ERR_NOMEM, "twiddle table for D=%d: %s", D, hipGetErrorString(e));
+        }
+    }
     *out = p;
     return CM21_OK;
 }
@@ -140,4 +154,98 @@ extern "C" int cm21_c2r(cm21_plan* plan, void* spec_in, float* out, void* stream
 namespace cm21 {
 int plan_dim(const cm21_plan* p) { return p ? p->D : 0; }
 int plan_batch(const cm21_plan* p) { return p ? p->batch : 0; }
+const float2* plan_twiddles(const cm21_plan* p) { return p ? p->twiddles : nullptr; }
 }  // namespace cm21
+
+// ------------------------------------------------------------------ profiler
+// Per-stage HIP event timing.  While enabled, every instrumented launch
+// records an event pair on its own stream; cm21_profile_collect() waits for
+// the recorded events, adds their elapsed times to per-stage totals and
+// returns them.  Events come from a pool and are reused.
+namespace {
+struct ProfPair {
+    int stage;
+    hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+unsigned g_prof_mask = 0u;  // bit s enables stage s
+std::vector<ProfPair> g_prof_pending;
+std::vector<hipEvent_t> g_prof_pool;
+
+hipEvent_t pool_get() {
+    if (!g_prof_pool.empty()) {
+        hipEvent_t e = g_prof_pool.back();
+        g_prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+}  // namespace
+
+namespace cm21 {
+ProfToken prof_begin(int stage, hipStream_t st) {
+    ProfToken t{-1, nullptr};
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (stage < 0 || stage >= CM21_NSTAGES || !((g_prof_mask >> stage) & 1u)) return t;
+    hipEvent_t a = pool_get();
+    if (!a) return t;
+    if (hipEventRecord(a, st) != hipSuccess) {
+        g_prof_pool.push_back(a);
+        return t;
+    }
+    t.stage = stage;
+    t.start = a;
+    return t;
+}
+
+void prof_end(const ProfToken& t, hipStream_t st) {
+    if (t.stage < 0) return;
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    hipEvent_t b = pool_get();
+    if (!b || hipEventRecord(b, st) != hipSuccess) {
+        g_prof_pool.push_back((hipEvent_t)t.start);
+        if (b) g_prof_pool.push_back(b);
+        return;
+    }
+    g_prof_pending.push_back({t.stage, (hipEvent_t)t.start, b});
+}
+}  // namespace cm21
+
+extern "C" int cm21_profile_enable(int stage_mask) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_mask = (unsigned)stage_mask;
+    return CM21_OK;
+}
+
+extern "C" int cm21_profile_collect(double* ms, int64_t* count, int nstages) {
+    CM21_REQUIRE(ms && count && nstages >= 1, "cm21_profile_collect: NULL argument");
+    std::vector<ProfPair> pend;
+    {
+        std::lock_guard<std::mutex> lk(g_prof_mu);
+        pend.swap(g_prof_pending);
+    }
+    for (int i = 0; i < nstages; ++i) {
+        ms[i] = 0.0;
+        count[i] = 0;
+    }
+    int status = CM21_OK;
+    for (auto& p : pend) {
+        float t = 0.0f;
+        hipError_t e = hipEventSynchronize(p.b);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, p.a, p.b);
+        if (e != hipSuccess && status == CM21_OK)
+            status = cm21::fail(CM21_ERR_HIP, "cm21_profile_collect: %s", hipGetErrorString(e));
+        if (p.stage < nstages) {
+            ms[p.stage] += (double)t;
+            count[p.stage] += 1;
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (auto& p : pend) {
+        g_prof_pool.push_back(p.a);
+        g_prof_pool.push_back(p.b);
+    }
+    return status;
+}

@@ -1,0 +1,55 @@
+// Internal interface of the fused, pruned transform passes (cm21_fused.hip)
+// used by the radius-loop driver.
+#pragma once
+
+#include "cm21_internal.hpp"
+
+namespace cm21 {
+
+struct FusedParams {
+    int m2max;      // modes with |n|^2 > m2max are zero
+    int m2lo;       // modes with |n|^2 >= m2lo take the exact float32 cut test
+    int bm;         // largest |b| with b^2 <= m2max (capped at D/2)
+    int cmax;       // largest c with c^2 <= m2max (capped at D/2)
+    int exact_cut;  // 1 when the k-space cut applies
+    float dk, R;    // for the exact test
+    float gcoef;    // Gaussian factor exp2(m2 * gcoef); 0 for none
+    float scale;    // 1/N times the growth factor of nonlinear input
+};
+
+struct LineEpilogue {
+    float deltac, denom;
+    double zeta, inv_zeta;
+    const double* sum;  // FP64 box sum of f_coll (nonlinear input), or NULL
+    float fc_mean;
+    unsigned long long* mask;  // ionization state, one bit per cell
+    float* out;                // final float box
+    double* partial;           // per-block FP64 partial sums
+    double* sum_out;           // sum mode: the box sum, written by the last block
+    unsigned int* ticket;      // sum mode: blocks finished (zero between launches)
+    double ncells;
+};
+
+struct FusedShape {
+    int D;
+    int (*columns)(const float2*, float2*, const float2*, const FusedParams&, int ncg, int nb, hipStream_t);
+    int (*rows)(float2*, const float2*, const FusedParams&, int ncg, hipStream_t);
+    int (*lines)(int mode, const float2*, const float2*, const FusedParams&, const LineEpilogue&, hipStream_t);
+    int W, L;
+};
+
+const FusedShape* fused_shape(int D);
+size_t fused_partials(int D);
+FusedParams fused_params(int D, float dk, float R, int filter_type, int ref_fallthrough, float scale);
+LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double* sum, float fc_mean,
+                           unsigned long long* mask, float* out, double* partial, double* sum_out,
+                           unsigned int* ticket, double ncells);
+int fused_ticket_clear(unsigned int* ticket, hipStream_t st);
+int fused_radius(const FusedShape& sh, const float2* S, float2* T, const float2* tw, const FusedParams& fp,
+                 hipStream_t st);
+int fused_lines(const FusedShape& sh, int mode, const float2* U, const float2* tw, const FusedParams& fp,
+                const LineEpilogue& ep, hipStream_t st);
+int fused_mask_clear(unsigned long long* mask, size_t words, hipStream_t st);
+int fused_sum(const double* partial, int n, double* out, hipStream_t st);
+
+}  // namespace cm21

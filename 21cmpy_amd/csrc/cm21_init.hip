@@ -135,7 +135,7 @@ __global__ void k_velocity(float2* __restrict__ out, const float2* __restrict__ 
         WindowParams wp{dk, R, filter_type, 1, 1};
         float w0, w2;
         bool cut;
-        window_of(__fsqrt_rn(ksq), wp, w0, cut, w2);
+        window_of(sqrt_rn(ksq), wp, w0, cut, w2);
         r = apply_window(r, w0, cut, w2, 1.0f);
     }
     out[idx] = r;

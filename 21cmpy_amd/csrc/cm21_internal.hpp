@@ -52,6 +52,15 @@ __device__ inline double block_sum(double v) {
     return r;
 }
 
+// Event-pair timing of instrumented launches (no-op unless enabled with
+// cm21_profile_enable).
+struct ProfToken {
+    int stage;
+    void* start;
+};
+ProfToken prof_begin(int stage, hipStream_t st);
+void prof_end(const ProfToken& t, hipStream_t st);
+
 // Signed mode number of FFT index i on an axis of length D; index D/2 maps
 // to +D/2 (Programs/find_bubbles.cu:22-24).
 __device__ inline int mode_of(int i, int D) { return (i > D / 2) ? i - D : i; }

@@ -17,6 +17,12 @@ struct WindowParams {
 
 constexpr float kE = 2.7182818284f;  // find_bubbles.cu:3, `(float)(2.7182818284)`
 
+// Correctly rounded float32 square root: the FP64 root of a float32 value
+// rounded once to float32 (53 >= 2*24+2 bits, so the double rounding is
+// innocuous).  The cut-off decision kR*0.413566994 > 1 depends on the last
+// bit of |k|.
+__device__ inline float sqrt_rn(float s) { return (float)__dsqrt_rn((double)s); }
+
 // |k| of half-spectrum element (a, b, c); axis 2 (c) is the halved axis.
 __device__ inline float kmag_of(int a, int b, int c, int D, float dk, int korder) {
     const float kz = __fmul_rn((float)mode_of(a, D), dk);
@@ -24,7 +30,7 @@ __device__ inline float kmag_of(int a, int b, int c, int D, float dk, int korder
     const float kx = __fmul_rn((float)c, dk);
     const float sx = __fmul_rn(kx, kx), sy = __fmul_rn(ky, ky), sz = __fmul_rn(kz, kz);
     const float s = korder == 0 ? __fadd_rn(__fadd_rn(sx, sy), sz) : __fadd_rn(__fadd_rn(sz, sy), sx);
-    return __fsqrt_rn(s);
+    return sqrt_rn(s);
 }
 
 // Evaluates the window for |k| = kmag.  The caller multiplies the spectrum

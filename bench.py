@@ -1,0 +1,238 @@
+"""Benchmark of the find_HII_bubbles hot path on one MI355X.
+
+Workload (the full plain pipeline at HII_DIM=256): a DIM=512, BOX_LEN=300 Mpc
+Gaussian random field (P(k) ~ k^-2.5 exp(-(k/k_nyq)^2), per-cell sigma 0.6,
+seed 0) is smoothed and nearest-picked to 256^3, moved by the Zel'dovich
+displacement at growth 0.12 (D_init = 0.12*9/301) with a cloud-in-cell
+deposit, and the resulting nonlinear 256^3 box goes through find_HII_bubbles:
+40 radii, zeta = 30, k-space top-hat (HII_FILTER = 1), box-mean
+normalisation, sigma_min^2 = 4.0, barrier 1.686/0.12.
+
+A step is one find_HII_bubbles pass over that box, with the box already in
+HBM.  The JSON line reports boxes/s, the roofline of the dominant stage
+(algorithmic bytes per launch over its HIP-event-timed average duration) and a
+CPU baseline: the NumPy FP32 oracle of the same algorithm on a bounded sample.
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def alg_bytes(D):
+    """Algorithmic bytes of each stage per launch (one radius), from the unit
+    model B_alg = 4N + 8Nc + n_R (24 Nc + (16 + 4 norm) N)."""
+    N = D ** 3
+    Nc = D * D * (D // 2 + 1)
+    return {
+        "r2c": 4 * N + 8 * Nc,
+        "filter": 16 * Nc,
+        "c2r": 8 * Nc + 4 * N,
+        "fcoll_sum": 4 * N,
+        "threshold": 12 * N,
+        "fused_columns": 16 * Nc,   # filter + first transform axis: read and write the spectrum
+        "fused_rows": 16 * Nc,      # second axis: read and write the spectrum
+        "fused_lines": 8 * Nc + 12 * N,  # last axis c2r + threshold: read spectrum, read/write state, ...
+    }
+
+
+def b_alg(D, n_r, norm):
+    N = D ** 3
+    Nc = D * D * (D // 2 + 1)
+    return 4 * N + 8 * Nc + n_r * (24 * Nc + (16 + 4 * norm) * N)
+
+
+def cpu_baseline(delta, L, radii, denoms, zeta, deltac, fc_mean, budget_s=15.0):
+    """NumPy FP32 restatement (the oracle) on the same box: time the forward
+    transform plus as many radii as fit the budget; extrapolate to one box."""
+    from oracle import bubbles
+    D = delta.shape[0]
+    t0 = time.perf_counter()
+    d = (delta * np.float32(deltac / 1.686)).astype(np.float32)
+    spec = np.fft.rfftn(d.astype(np.float64)).astype(np.complex64)
+    km = bubbles.kmag_half_f32(D, bubbles.delta_k_f32(L))
+    t_fwd = time.perf_counter() - t0
+    done, t_rad = 0, 0.0
+    ion = np.zeros(delta.shape, np.float32)
+    for R, den in zip(radii, denoms):
+        t1 = time.perf_counter()
+        w0, cut, w2 = bubbles.window_f32(km, R, 1, True)
+        filt = bubbles.apply_window_f32(spec, w0, cut, w2)
+        sm = np.fft.irfftn(filt.astype(np.complex128), s=(D, D, D), axes=(0, 1, 2)).astype(np.float32)
+        fc = bubbles.fcoll_f32(sm, deltac, den)
+        fc = (fc * np.float32(float(fc_mean) / (fc.astype(np.float64).sum() / D ** 3))).astype(np.float32)
+        ion[fc.astype(np.float64) >= 1.0 / zeta] = 1.0
+        t_rad += time.perf_counter() - t1
+        done += 1
+        if t_fwd + t_rad > budget_s:
+            break
+    per_box = t_fwd + t_rad / done * len(radii)
+    return {"value": 1.0 / per_box, "unit": "boxes/s", "cores": 1, "kind": "port",
+            "sample": f"NumPy FP32 oracle (oracle/bubbles.py, pocketfft single thread) on the same {D}^3 box: "
+                      f"forward transform + {done} of {len(radii)} radii timed ({t_fwd + t_rad:.1f} s), "
+                      f"extrapolated to one box"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--generic", action="store_true", help="rocFFT c2r + separate kernels instead of the fused path")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(independent boxes shard across GPUs with no exchange; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    lib = importlib.import_module("21cmpy_amd._lib")
+    init = importlib.import_module("21cmpy_amd.Programs.initialize")
+    pert = importlib.import_module("21cmpy_amd.Programs.perturb_field")
+    fb = importlib.import_module("21cmpy_amd.Programs.find_HII_bubbles")
+    dt = importlib.import_module("21cmpy_amd.Programs.delta_T")
+    sig = importlib.import_module("21cmpy_amd.cosmo_files.sigmas")
+
+    DIM, D, L, seed = 512, 256, 300.0, 0
+    growth, zeta, smin = 0.12, 30.0, 4.0
+    growth_init = growth * 9.0 / 301.0
+    deltac = 1.686 / growth
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, (time.perf_counter() - t) * 1e3
+
+    # pipeline stages (once; the first call of each includes plan creation)
+    ics, _ = timed(lambda: init.init(DIM=DIM, HII_DIM=D, BOX_LEN=L, seed=seed, save=False))
+    ics, t_init = timed(lambda: init.init(DIM=DIM, HII_DIM=D, BOX_LEN=L, seed=seed, save=False))
+    upd, _ = timed(lambda: pert.evolve_zeldovich(8.0, ics["deltax"], ics["v"], growth, growth_init, L, "cic"))
+    upd, t_pert = timed(lambda: pert.evolve_zeldovich(8.0, ics["deltax"], ics["v"], growth, growth_init, L, "cic"))
+    del ics
+    torch.cuda.empty_cache()
+
+    spectrum = sig.PowerLawSpectrum(D, L)
+    radii, denoms, fc_mean = fb.bubble_setup(D, L, None, deltac, smin, spectrum.sig0_discrete)
+    prm = lib.BubbleParams(box_len=L, deltac=float(np.float32(deltac)), zeta=zeta,
+                           growth_inv=float(np.float32(deltac / 1.686)), fc_mean=float(fc_mean), linear_input=0,
+                           filter_type=1, ref_fallthrough=1, update=0, sphere_periodic=0, sphere_r_scale=1.0,
+                           fused=0 if args.generic else 1)
+    plan = lib.plan(D, 1)
+    nbytes = lib.query("cm21_find_bubbles_workspace_bytes", D, len(radii), 0)
+    work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    ion = torch.empty_like(upd)
+    r_arr = np.ascontiguousarray(radii, np.float32)
+    d_arr = np.ascontiguousarray(denoms, np.float32)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        lib.call("cm21_find_bubbles", plan.handle, lib.ptr(upd), lib.ptr(ion), r_arr.ctypes.data_as(lib.c_float_p),
+                 d_arr.ctypes.data_as(lib.c_float_p), len(radii), prm, lib.ptr(work), nbytes,
+                 lib.stream_handle(stream))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    # calibration: every stage instrumented, to find the dominant one
+    lib.profile_collect()
+    lib.profile_enable(True)
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    lib.profile_enable(False)
+    calib = lib.profile_collect()
+    dom = max(calib, key=lambda k: calib[k][0])
+    stages_all = {k: {"ms_per_launch": ms / n, "launches_per_box": n / 2} for k, (ms, n) in calib.items()}
+
+    # the same steps with no events at all, to show what the instrumentation costs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms_no_events = (time.perf_counter() - t0) * 1e3 / args.steps
+
+    # timed region: only the dominant stage records HIP events
+    lib.profile_enable([dom])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lib.profile_enable(False)
+    prof = lib.profile_collect()
+    ms_step = elapsed * 1e3 / args.steps
+    boxes_s = args.steps / elapsed
+    x_HI = 1.0 - float(ion.double().mean())
+    ab = alg_bytes(D)
+    for k, v in stages_all.items():
+        v["gbs_alg"] = (ab[k] / (v["ms_per_launch"] * 1e-3) / 1e9) if k in ab else None
+    dom_ms = prof[dom][0] / prof[dom][1]
+    achieved = ab[dom] / (dom_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_D{D}.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("bytes_per_launch")
+
+    # delta_T and P21(k) in 40 log bins k_fund..k_nyq on the result (once)
+    def post():
+        xH = 1.0 - ion
+        dT, s = dt.brightness_temperature(xH, upd, 8.0)
+        return dt.power_spectrum(dT, L, dt.log_edges(L, D, 40), field_sum=s)
+    _, t_post = timed(post)
+    _, t_post = timed(post)
+
+    out = {
+        "metric": "HII boxes/s at HII_DIM=256: plain vs batched vs surrogate @ |dx_HI|<0.01",
+        "value": boxes_s,
+        "unit": "boxes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "ms_per_step_without_events": ms_no_events,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain find_HII_bubbles, HII_DIM=256 nonlinear box from DIM=512 GRF + CIC "
+                               "Zel'dovich move (BOX_LEN=300 Mpc, 40 radii, zeta=30, HII_FILTER=1)",
+                   "HII_DIM": D, "DIM": DIM, "BOX_LEN": L, "n_radii": len(radii), "zeta": zeta,
+                   "sigma_min2": smin, "growth": growth,
+                   "path": "plain-generic" if args.generic else "plain-fused"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": ab[dom], "ms_per_launch": dom_ms},
+        "model_roofline": {"B_alg_per_box": b_alg(D, len(radii), 1),
+                           "frac": b_alg(D, len(radii), 1) * boxes_s / (HBM_PEAK_GBS * 1e9)},
+        "stages": stages_all,
+        "pipeline_ms": {"initialize_512_to_256": t_init, "perturb_cic": t_pert, "delta_T_and_P21": t_post},
+        "x_HI": x_HI,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(upd.cpu().numpy(), L, radii, denoms, zeta, deltac, fc_mean,
+                                           args.cpu_budget)
+        out["cpu_baseline"]["cores_available"] = len(os.sched_getaffinity(0))
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

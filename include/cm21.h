@@ -125,6 +125,9 @@ typedef struct cm21_bubble_params {
     int update;           /* 0 = centre pixel, 1 = sphere paint */
     int sphere_periodic;  /* sphere paint wraps (1) or clips (0) */
     float sphere_r_scale; /* R_cells = R * sphere_r_scale (1 = reference) */
+    int fused;            /* 1 = fused pruned transforms where supported (update 0,
+                             filter_type 1 or 2, D in {64,96,128,160,256,512});
+                             0 = rocFFT c2r + separate filter/threshold kernels */
 } cm21_bubble_params;
 
 size_t cm21_find_bubbles_workspace_bytes(int D, int n_radii, int update);
@@ -186,6 +189,27 @@ int cm21_power_spectrum(cm21_plan* plan, const float* field,
                         const int32_t* bin_lut, int nbins, int dimensional,
                         double* ps_sum, double* k_sum, uint64_t* counts,
                         void* work, size_t work_bytes, void* stream);
+
+/* ------------------------------------------------------------------ profiler
+ * No reference counterpart (the reference records CUDA events it never reads,
+ * Programs/find_HII_bubbles.py:133-177).  cm21_profile_enable(mask) makes the
+ * launches of every stage s with bit s of mask set record a HIP event pair on
+ * their stream (mask 0 turns profiling off, -1 enables all stages); collect
+ * waits for the recorded events and returns per-stage total milliseconds and
+ * launch counts (arrays of nstages entries, indexed by CM21_STAGE_*), then
+ * clears the record. */
+#define CM21_STAGE_R2C 0
+#define CM21_STAGE_FILTER 1
+#define CM21_STAGE_C2R 2
+#define CM21_STAGE_FCOLL_SUM 3
+#define CM21_STAGE_THRESHOLD 4
+#define CM21_STAGE_SPHERE 5
+#define CM21_STAGE_FUSED_COLUMNS 6
+#define CM21_STAGE_FUSED_ROWS 7
+#define CM21_STAGE_FUSED_LINES 8
+#define CM21_NSTAGES 9
+int cm21_profile_enable(int stage_mask);
+int cm21_profile_collect(double* ms, int64_t* count, int nstages);
 
 #ifdef __cplusplus
 }

@@ -29,13 +29,14 @@ def _compare(got, want, margin):
     assert np.mean(ok) > 0.99
 
 
-def _run_gpu(delta, L, smin, zeta, LE, deltac, fil=1, strict=False, update=0, periodic=False, in_cells=False):
+def _run_gpu(delta, L, smin, zeta, LE, deltac, fil=1, strict=False, update=0, periodic=False, in_cells=False,
+             fused=True):
     fb = pkg("Programs.find_HII_bubbles")
     D = delta.shape[0]
     return fb.conv_bubbles(delta, {"BoxSize": L, "HIIdim": D}, fil=fil, update=update, LE=LE, quiet=True,
                            deltac=deltac, sigma_min2=smin, zeta=zeta,
                            sig0=lambda R: sigma.sig0_discrete(R, D, L), strict_filter=strict,
-                           sphere_periodic=periodic, sphere_in_cells=in_cells)
+                           sphere_periodic=periodic, sphere_in_cells=in_cells, fused=fused)
 
 
 def _oracle(delta, L, smin, zeta, LE, deltac, fil=1, strict=False, update=0, periodic=False, in_cells=False):
@@ -74,18 +75,20 @@ def test_config1_literal_parameters(gpu):
     assert np.all(got == 1) and np.all(want == 1)
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("fil,strict", [(1, False), (0, False), (2, False), (0, True), (1, True), (2, True)])
-def test_loop_filters_linear(gpu, fil, strict):
+def test_loop_filters_linear(gpu, fil, strict, fused):
     D, L = 64, 100.0
     delta = grf.c2r(grf.grf_half_spectrum(D, L, 1), D).astype(np.float32)
-    got = _run_gpu(delta, L, 0.5, 30.0, True, 1.686, fil, strict)
+    got = _run_gpu(delta, L, 0.5, 30.0, True, 1.686, fil, strict, fused=fused)
     want, margin = _oracle(delta, L, 0.5, 30.0, True, 1.686, fil, strict)
     assert 0.05 < np.mean(want == 1) < 0.95
     _compare(got, want, margin)
 
 
-@pytest.mark.parametrize("D,L", [(64, 100.0), (96, 150.0), (128, 200.0)])
-def test_loop_nonlinear(gpu, D, L):
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("D,L", [(64, 100.0), (96, 150.0), (128, 200.0), (160, 250.0)])
+def test_loop_nonlinear(gpu, D, L, fused):
     growth = 0.12
     lin = grf.c2r(grf.grf_half_spectrum(D, L, 2), D).astype(np.float32)
     delta = (lin * np.float32(growth)).astype(np.float32)
@@ -93,7 +96,7 @@ def test_loop_nonlinear(gpu, D, L):
     # the loop extrapolates delta to z=0 (x 1/D) and uses the z=0 barrier
     # 1.686/D; sigma_min^2 in z=0 units then sits near (deltac/1.5)^2/2
     smin = 44.0
-    got = _run_gpu(delta, L, smin, 30.0, False, deltac)
+    got = _run_gpu(delta, L, smin, 30.0, False, deltac, fused=fused)
     want, margin = _oracle(delta, L, smin, 30.0, False, deltac)
     assert 0.05 < np.mean(want == 1) < 0.95
     _compare(got, want, margin)
@@ -125,6 +128,41 @@ def test_loop_deterministic_and_device_io(gpu):
     assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("D,L", [(128, 200.0), (256, 300.0)])
+def test_fused_equals_generic(gpu, D, L):
+    # The fused transforms and the rocFFT path agree flag for flag away from
+    # the threshold band and to 1e-4 on partial values (both FP32).
+    fb = pkg("Programs.find_HII_bubbles")
+    g = pkg("gpu")
+    sig = pkg("cosmo_files.sigmas")
+    sp = sig.PowerLawSpectrum(D, L)
+    delta = g.c2r(g.grf_init(D, L, sp.amp, 3))
+    for LE, deltac, smin in ((True, 1.686, 0.5), (False, 1.686 / 0.5, 2.5)):
+        d = delta if LE else delta * 0.5
+        kw = dict(LE=LE, quiet=True, deltac=deltac, sigma_min2=smin, zeta=30.0, sig0=sp.sig0_discrete)
+        a = fb.conv_bubbles(d, {"BoxSize": L, "HIIdim": D}, fused=True, **kw)
+        b = fb.conv_bubbles(d, {"BoxSize": L, "HIIdim": D}, fused=False, **kw)
+        diff = float(((a == 1) != (b == 1)).float().mean())
+        assert diff < 2e-5, diff
+        part = (a < 1) & (b < 1)
+        assert bool(torch.allclose(a[part], b[part], rtol=1e-4, atol=1e-6))
+
+
+@pytest.mark.parametrize("D", [512])
+def test_fused_large_box_runs(gpu, D):
+    fb = pkg("Programs.find_HII_bubbles")
+    g = pkg("gpu")
+    sig = pkg("cosmo_files.sigmas")
+    L = 500.0
+    sp = sig.PowerLawSpectrum(D, L)
+    delta = g.c2r(g.grf_init(D, L, sp.amp, 0))
+    kw = dict(LE=True, quiet=True, deltac=1.686, sigma_min2=0.5, zeta=25.0, sig0=sp.sig0_discrete)
+    a = fb.conv_bubbles(delta, {"BoxSize": L, "HIIdim": D}, fused=True, **kw)
+    b = fb.conv_bubbles(delta, {"BoxSize": L, "HIIdim": D}, fused=False, **kw)
+    assert float(((a == 1) != (b == 1)).float().mean()) < 2e-5
+    assert 0.05 < float((a == 1).float().mean()) < 0.95
+
+
 def test_full_size_properties(gpu):
     # D=256, L=300 (40 radii): size-independent properties of the result.
     fb = pkg("Programs.find_HII_bubbles")
@@ -146,3 +184,33 @@ def test_full_size_properties(gpu):
     full = fb.conv_bubbles(delta, {"BoxSize": L, "HIIdim": D}, LE=True, quiet=True, deltac=1.686, sigma_min2=4.0,
                            zeta=30.0, sig0=sp.sig0_discrete)
     assert bool(torch.all(full == 1))
+
+
+@pytest.mark.parametrize("offset", [3e-6, -3e-6, 2e-4, -2e-4, 5e-3, -5e-3])
+@pytest.mark.parametrize("LE", [True, False])
+def test_threshold_near_band_constant_box(gpu, offset, LE):
+    # A constant box keeps only the DC mode, so delta_R = c at every radius.
+    # Put zeta*f_coll at the last radius a relative `offset` away from 1:
+    # offsets of 3e-6 fall inside the exact-test band of the density-space
+    # threshold, 2e-4 near its edge and 5e-3 outside it.  Decisions must match
+    # the FP32 oracle; partial values within 1e-4.
+    from scipy.special import erfcinv
+    D, L = 64, 100.0
+    radii, _ = sigma.radius_schedule(L, D)
+    s0 = lambda R: sigma.sig0_discrete(R, D, L)  # noqa: E731
+    smin = 0.5
+    den = sigma.denominators(radii, smin, s0)
+    deltac = 1.686
+    zeta = 30.0
+    target = (1.0 + offset) / zeta
+    c = deltac - float(den[-1]) * float(erfcinv(target))
+    growth = 1.0 if LE else 0.5
+    dc = deltac if LE else deltac / growth
+    c_in = c * growth
+    delta = np.full((D, D, D), c_in, np.float32)
+    got = _run_gpu(delta, L, smin, zeta, LE, dc)
+    want, margin = _oracle(delta, L, smin, zeta, LE, dc)
+    if np.all(margin > FLAG_BAND):
+        assert np.array_equal(got == 1, want == 1)
+    part = (want < 1) & (got < 1)
+    assert np.allclose(got[part], want[part], rtol=1e-4)
