@@ -89,7 +89,7 @@ def bubble_setup(D, box_len, scale, deltac, sigma_min2, sig0_fn, py2_int_scale=F
 def conv_bubbles(deltax, param_dict, Z=None, scale=None, fil=1, update=0, LE=False, visualize=0, quiet=False,
                  *, deltac=None, growth=None, sigma_min2=None, zeta=None, box_len=None, sig0=None,
                  strict_filter=False, py2_int_scale=False, sphere_periodic=False, sphere_in_cells=False,
-                 fused=True, stream=None):
+                 fused=True, graph=False, stream=None):
     """Ionized box from a density box by the excursion-set radius loop.
 
     deltax: float32 (D,D,D) NumPy array or CUDA tensor (returned in the same
@@ -102,6 +102,8 @@ def conv_bubbles(deltax, param_dict, Z=None, scale=None, fil=1, update=0, LE=Fal
     fused: use the fused, pruned transform kernels where supported (centre-pixel
     update, filter 1 or 2, D in 64/96/128/160/256/512); False forces rocFFT plus
     separate filter and threshold kernels.
+    graph: capture the loop into a HIP graph and replay it when called again
+    with the same tensors and parameters (needs a non-default stream).
     """
     if visualize:
         raise ValueError("visualize is not supported by the GPU path")
@@ -130,23 +132,39 @@ def conv_bubbles(deltax, param_dict, Z=None, scale=None, fil=1, update=0, LE=Fal
         fc_mean=float(fc_mean), linear_input=1 if LE else 0, filter_type=int(fil),
         ref_fallthrough=0 if strict_filter else 1, update=int(update),
         sphere_periodic=1 if sphere_periodic else 0,
-        sphere_r_scale=(1.0 / cell) if sphere_in_cells else 1.0, fused=1 if fused else 0)
+        sphere_r_scale=(1.0 / cell) if sphere_in_cells else 1.0, fused=1 if fused else 0,
+        graph=1 if graph else 0)
     is_tensor = isinstance(deltax, torch.Tensor)
     dev = torch.device("cuda")
     d = deltax if is_tensor else torch.from_numpy(np.ascontiguousarray(deltax, dtype=np.float32))
     d = d.to(dev, dtype=torch.float32).contiguous()
-    ion = torch.empty_like(d)
     plan = _lib.plan(D, 1)
     nbytes = _lib.query("cm21_find_bubbles_workspace_bytes", D, len(radii), int(update))
-    work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    if graph:
+        # a replayed graph reuses the captured addresses: keep the buffers
+        ion, work = _graph_buffers(d, nbytes)
+    else:
+        ion = torch.empty_like(d)
+        work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     r = np.ascontiguousarray(radii, dtype=np.float32)
     dn = np.ascontiguousarray(denoms, dtype=np.float32)
     _lib.call("cm21_find_bubbles", plan.handle, _lib.ptr(d), _lib.ptr(ion),
               r.ctypes.data_as(_lib.c_float_p), dn.ctypes.data_as(_lib.c_float_p), len(radii),
               prm, _lib.ptr(work), nbytes, _lib.stream_handle(stream))
     if is_tensor:
-        return ion
+        return ion.clone() if graph else ion
     return ion.cpu().numpy()
+
+
+_graph_cache = {}
+
+
+def _graph_buffers(d, nbytes):
+    """Output and workspace kept alive per (input address, shape) for graph replay."""
+    key = (d.data_ptr(), tuple(d.shape), nbytes)
+    if key not in _graph_cache:
+        _graph_cache[key] = (torch.empty_like(d), torch.empty(nbytes, dtype=torch.uint8, device=d.device))
+    return _graph_cache[key]
 
 
 def xH_filename(Z, nf, fil, dim, box_size):

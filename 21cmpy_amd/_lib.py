@@ -44,6 +44,7 @@ class BubbleParams(ctypes.Structure):
         ("sphere_periodic", ctypes.c_int),
         ("sphere_r_scale", ctypes.c_float),
         ("fused", ctypes.c_int),
+        ("graph", ctypes.c_int),
     ]
 
 

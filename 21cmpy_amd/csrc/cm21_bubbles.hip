@@ -10,6 +10,10 @@
 //  * the box mean of f_coll is an FP64 device scalar read by the threshold
 //    kernel, so the loop never synchronises with the host.
 #include <cmath>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
 
 #include "cm21_internal.hpp"
 #include "cm21_window.hpp"
@@ -246,7 +250,6 @@ struct LoopWork {
     void* sphere;
     unsigned long long* mask;
     double* lpart;
-    unsigned int* ticket;
     size_t bytes;
 };
 
@@ -272,11 +275,16 @@ LoopWork carve(void* base, int D, int n_radii, int update) {
     off = align_up(off + sizeof(unsigned long long) * ((N + 63) / 64), 256);
     w.lpart = (double*)(p + off);
     off = align_up(off + sizeof(double) * (fused_partials(D) + 1), 256);
-    w.ticket = (unsigned int*)(p + off);
-    off = align_up(off + sizeof(unsigned int), 256);
     w.bytes = off;
     return w;
 }
+}  // namespace
+
+namespace {
+int run_loop(cm21_plan* plan, const float* delta, float* ionized, const float* radii, const float* denoms,
+             int n_radii, const cm21_bubble_params* prm, const LoopWork& w, hipStream_t st);
+int graph_loop(cm21_plan* plan, const float* delta, float* ionized, const float* radii, const float* denoms,
+               int n_radii, const cm21_bubble_params* prm, const LoopWork& w, hipStream_t st);
 }  // namespace
 
 extern "C" size_t cm21_find_bubbles_workspace_bytes(int D, int n_radii, int update) {
@@ -310,6 +318,15 @@ extern "C" int cm21_find_bubbles(cm21_plan* plan, const float* delta, float* ion
         return cm21::fail(CM21_ERR_NOMEM, "cm21_find_bubbles: workspace %zu bytes < required %zu", work_bytes,
                           w.bytes);
     hipStream_t st = (hipStream_t)stream;
+    if (prm->graph && st != nullptr && !prof_any()) return graph_loop(plan, delta, ionized, radii, denoms, n_radii, prm, w, st);
+    return run_loop(plan, delta, ionized, radii, denoms, n_radii, prm, w, st);
+}
+
+namespace {
+int run_loop(cm21_plan* plan, const float* delta, float* ionized, const float* radii, const float* denoms,
+             int n_radii, const cm21_bubble_params* prm, const LoopWork& w, hipStream_t st) {
+    void* stream = (void*)st;
+    const int D = plan_dim(plan);
     const size_t N = (size_t)D * D * D;
 
     const float dk = (float)(2.0 * M_PI / (double)prm->box_len);
@@ -323,20 +340,23 @@ extern "C" int cm21_find_bubbles(cm21_plan* plan, const float* delta, float* ion
     if (prm->fused && sh && prm->update == 0 && (prm->filter_type == 1 || prm->filter_type == 2)) {
         const float2* tw = plan_twiddles(plan);
         CM21_TRY(fused_mask_clear(w.mask, (N + 63) / 64, st));
-        CM21_TRY(fused_ticket_clear(w.ticket, st));
         for (int r = 0; r < n_radii; ++r) {
             const bool final_step = r == n_radii - 1;
             FusedParams fp = fused_params(D, dk, radii[r], prm->filter_type, prm->ref_fallthrough, scale);
             CM21_TRY(fused_radius(*sh, w.spec, w.filt, tw, fp, st));
+            // Nonlinear input: a sum pass for the FP64 box mean, then the
+            // threshold pass recomputes the (cheap) last-axis transform rather
+            // than storing and re-reading the smoothed field.
             const double* sum = nullptr;
             if (!prm->linear_input) {
                 LineEpilogue es = line_epilogue(prm->deltac, denoms[r], prm->zeta, nullptr, prm->fc_mean, w.mask,
-                                                ionized, w.lpart, w.sums + r, w.ticket, (double)N);
+                                                ionized, w.lpart, (double)N);
                 CM21_TRY(fused_lines(*sh, 0, w.filt, tw, fp, es, st));
+                CM21_TRY(fused_sum(w.lpart, (int)fused_partials(D), w.sums + r, st));
                 sum = w.sums + r;
             }
             LineEpilogue ep = line_epilogue(prm->deltac, denoms[r], prm->zeta, sum, prm->fc_mean, w.mask, ionized,
-                                            w.lpart, nullptr, w.ticket, (double)N);
+                                            w.lpart, (double)N);
             CM21_TRY(fused_lines(*sh, final_step ? 2 : 1, w.filt, tw, fp, ep, st));
         }
         return CM21_OK;
@@ -373,3 +393,74 @@ extern "C" int cm21_find_bubbles(cm21_plan* plan, const float* delta, float* ion
     }
     return CM21_OK;
 }
+
+// Graph replay of the whole loop.  The first call with a given set of
+// arguments (pointers, parameters, radii, denominators, stream) captures the
+// launches into a HIP graph; identical later calls replay it, which removes
+// the host launch cost of the ~5 kernels per radius.  One graph is kept per
+// plan.  A capture that fails falls back to eager launches.
+struct GraphKey {
+    const void* delta;
+    const void* ionized;
+    const void* work;
+    hipStream_t stream;
+    cm21_bubble_params prm;
+    std::vector<float> radii, denoms;
+    bool operator==(const GraphKey& o) const {
+        return delta == o.delta && ionized == o.ionized && work == o.work && stream == o.stream &&
+               std::memcmp(&prm, &o.prm, sizeof(prm)) == 0 && radii == o.radii && denoms == o.denoms;
+    }
+};
+struct GraphEntry {
+    GraphKey key;
+    hipGraphExec_t exec = nullptr;
+};
+std::mutex g_graph_mu;
+std::unordered_map<const cm21_plan*, GraphEntry> g_graphs;
+
+int graph_loop(cm21_plan* plan, const float* delta, float* ionized, const float* radii, const float* denoms,
+               int n_radii, const cm21_bubble_params* prm, const LoopWork& w, hipStream_t st) {
+    GraphKey key{delta, ionized, w.spec, st, *prm, std::vector<float>(radii, radii + n_radii),
+                 std::vector<float>(denoms, denoms + n_radii)};
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    auto it = g_graphs.find(plan);
+    if (it != g_graphs.end() && it->second.exec && it->second.key == key) {
+        CM21_HIP_CHECK(hipGraphLaunch(it->second.exec, st));
+        return CM21_OK;
+    }
+    if (it != g_graphs.end() && it->second.exec) {
+        (void)hipGraphExecDestroy(it->second.exec);
+        it->second.exec = nullptr;
+    }
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return run_loop(plan, delta, ionized, radii, denoms, n_radii, prm, w, st);
+    }
+    const int rc = run_loop(plan, delta, ionized, radii, denoms, n_radii, prm, w, st);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    hipGraphExec_t exec = nullptr;
+    bool ok = rc == CM21_OK && ec == hipSuccess && graph;
+    if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!ok) {
+        (void)hipGetLastError();
+        if (rc != CM21_OK) return rc;
+        return run_loop(plan, delta, ionized, radii, denoms, n_radii, prm, w, st);
+    }
+    g_graphs[plan] = GraphEntry{std::move(key), exec};
+    CM21_HIP_CHECK(hipGraphLaunch(exec, st));
+    return CM21_OK;
+}
+}  // namespace
+
+namespace cm21 {
+void forget_plan_graphs(const cm21_plan* plan) {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    auto it = g_graphs.find(plan);
+    if (it != g_graphs.end()) {
+        if (it->second.exec) (void)hipGraphExecDestroy(it->second.exec);
+        g_graphs.erase(it);
+    }
+}
+}  // namespace cm21

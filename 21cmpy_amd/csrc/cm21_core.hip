@@ -59,6 +59,7 @@ extern "C" const char* cm21_last_error(void) { return g_last_error.c_str(); }
 
 static void plan_release(cm21_plan* p) {
     if (!p) return;
+    cm21::forget_plan_graphs(p);
     if (p->fwd) rocfft_plan_destroy(p->fwd);
     if (p->inv) rocfft_plan_destroy(p->inv);
     if (p->info) rocfft_execution_info_destroy(p->info);
@@ -198,6 +199,11 @@ ProfToken prof_begin(int stage, hipStream_t st) {
     t.stage = stage;
     t.start = a;
     return t;
+}
+
+bool prof_any() {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    return g_prof_mask != 0u;
 }
 
 void prof_end(const ProfToken& t, hipStream_t st) {

@@ -163,9 +163,9 @@ __device__ inline DensityThreshold density_threshold(float deltac, float denom, 
 // hold x[2m] (real part) and x[2m+1] (imaginary part).  One LDS buffer holds,
 // in turn, the half lines, the transposed partial transforms and the real
 // outputs; rows are padded to an odd number of float2 so that lanes walking
-// different lines hit different banks.  In sum mode the last block to finish
-// (integer ticket) adds the per-block partials in a fixed order, so the FP64
-// box sum is deterministic and needs no second launch.
+// different lines hit different banks.  In sum mode each block writes its FP64
+// partial; a one-block kernel adds them in a fixed order (deterministic), and
+// the kernel boundary makes them visible across the chip's L2 slices.
 template <int N1, int N2, int L, int MODE>
 __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict__ U, const float2* __restrict__ tw,
                                                       FusedParams fp, LineEpilogue ep) {
@@ -179,7 +179,6 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
     __shared__ float2 buf[L * HP];
     __shared__ float2 stw[D];
     __shared__ unsigned long long smask[WORDS];
-    __shared__ int last_block;
     const int t = threadIdx.x;
     const size_t line0 = (size_t)blockIdx.x * L;
     const size_t word0 = line0 * D / 64;
@@ -204,7 +203,7 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
     if (MODE != kLineSum && t < WORDS) smask[t] = mword;
     __syncthreads();
     {
-        const int i = t / N1, n1 = t % N1;
+        const int i = t % L, n1 = t / L;  // lanes walk lines: distinct banks
         float2 v[N2];
 #pragma unroll
         for (int n2 = 0; n2 < N2; ++n2) {
@@ -247,8 +246,10 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
     const size_t cell0 = line0 * D;
     float scale = 1.0f;
     if (MODE != kLineSum && ep.sum) scale = (float)((double)ep.fc_mean / (*ep.sum / ep.ncells));
-    DensityThreshold th{0.0f, 0.0f};
-    if (MODE != kLineSum) th = density_threshold(ep.deltac, ep.denom, ep.zeta, scale);
+    __shared__ DensityThreshold sth;
+    if (MODE != kLineSum && t == 0) sth = density_threshold(ep.deltac, ep.denom, ep.zeta, scale);
+    __syncthreads();
+    const DensityThreshold th = sth;
     double acc = 0.0;
     const int lane = t & 63;
     const float* xrf = reinterpret_cast<const float*>(buf);
@@ -285,23 +286,7 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
     }
     if (MODE == kLineSum) {
         acc = block_sum(acc);
-        if (t == 0) {
-            ep.partial[blockIdx.x] = acc;
-            __threadfence();
-            last_block = atomicAdd(ep.ticket, 1u) == gridDim.x - 1;
-        }
-        __syncthreads();
-        if (last_block) {
-            __threadfence();
-            double tot = 0.0;
-            for (int q = t; q < (int)gridDim.x; q += NT)
-                tot += __hip_atomic_load(&ep.partial[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            tot = block_sum(tot);
-            if (t == 0) {
-                *ep.sum_out = tot;
-                *ep.ticket = 0u;  // ready for the next radius
-            }
-        }
+        if (t == 0) ep.partial[blockIdx.x] = acc;
     }
     if (MODE == kLineThreshold) {
         __syncthreads();
@@ -314,7 +299,6 @@ __global__ void k_fill_u64(unsigned long long* __restrict__ p, size_t n) {
     if (i < n) p[i] = 0ull;
 }
 
-__global__ void k_zero_ticket(unsigned int* __restrict__ p) { *p = 0u; }
 
 __global__ void k_sum_partials_lines(const double* __restrict__ partial, int n, double* __restrict__ out) {
     double acc = 0.0;
@@ -361,7 +345,7 @@ const FusedShape kShapes[] = {
     {96, launch_columns<8, 12, 32>, launch_rows<8, 12, 32>, launch_lines<4, 12, 64>, 32, 64},
     {128, launch_columns<8, 16, 32>, launch_rows<8, 16, 32>, launch_lines<8, 8, 32>, 32, 32},
     {160, launch_columns<16, 10, 16>, launch_rows<16, 10, 16>, launch_lines<8, 10, 32>, 16, 32},
-    {256, launch_columns<16, 16, 16>, launch_rows<16, 16, 16>, launch_lines<8, 16, 32>, 16, 32},
+    {256, launch_columns<16, 16, 16>, launch_rows<16, 16, 16>, launch_lines<16, 8, 16>, 16, 16},
     {512, launch_columns<16, 32, 8>, launch_rows<16, 32, 8>, launch_lines<16, 16, 8>, 8, 8},
 };
 
@@ -446,8 +430,7 @@ int fused_mask_clear(unsigned long long* mask, size_t words, hipStream_t st) {
 }
 
 LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double* sum, float fc_mean,
-                           unsigned long long* mask, float* out, double* partial, double* sum_out,
-                           unsigned int* ticket, double ncells) {
+                           unsigned long long* mask, float* out, double* partial, double ncells) {
     LineEpilogue ep{};
     ep.deltac = deltac;
     ep.denom = denom;
@@ -458,16 +441,8 @@ LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double*
     ep.mask = mask;
     ep.out = out;
     ep.partial = partial;
-    ep.sum_out = sum_out;
-    ep.ticket = ticket;
     ep.ncells = ncells;
     return ep;
-}
-
-int fused_ticket_clear(unsigned int* ticket, hipStream_t st) {
-    k_zero_ticket<<<1, 1, 0, st>>>(ticket);
-    CM21_LAUNCH_CHECK("k_zero_ticket");
-    return CM21_OK;
 }
 
 int fused_sum(const double* partial, int n, double* out, hipStream_t st) {

@@ -25,8 +25,6 @@ struct LineEpilogue {
     unsigned long long* mask;  // ionization state, one bit per cell
     float* out;                // final float box
     double* partial;           // per-block FP64 partial sums
-    double* sum_out;           // sum mode: the box sum, written by the last block
-    unsigned int* ticket;      // sum mode: blocks finished (zero between launches)
     double ncells;
 };
 
@@ -42,9 +40,7 @@ const FusedShape* fused_shape(int D);
 size_t fused_partials(int D);
 FusedParams fused_params(int D, float dk, float R, int filter_type, int ref_fallthrough, float scale);
 LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double* sum, float fc_mean,
-                           unsigned long long* mask, float* out, double* partial, double* sum_out,
-                           unsigned int* ticket, double ncells);
-int fused_ticket_clear(unsigned int* ticket, hipStream_t st);
+                           unsigned long long* mask, float* out, double* partial, double ncells);
 int fused_radius(const FusedShape& sh, const float2* S, float2* T, const float2* tw, const FusedParams& fp,
                  hipStream_t st);
 int fused_lines(const FusedShape& sh, int mode, const float2* U, const float2* tw, const FusedParams& fp,

@@ -60,6 +60,9 @@ struct ProfToken {
 };
 ProfToken prof_begin(int stage, hipStream_t st);
 void prof_end(const ProfToken& t, hipStream_t st);
+bool prof_any();  // any stage instrumented
+
+void forget_plan_graphs(const ::cm21_plan* plan);
 
 // Signed mode number of FFT index i on an axis of length D; index D/2 maps
 // to +D/2 (Programs/find_bubbles.cu:22-24).

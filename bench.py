@@ -28,9 +28,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
-def alg_bytes(D):
-    """Algorithmic bytes of each stage per launch (one radius), from the unit
-    model B_alg = 4N + 8Nc + n_R (24 Nc + (16 + 4 norm) N)."""
+def alg_bytes(D, norm=1):
+    """Algorithmic bytes per launch (one radius) of each stage, from the unit
+    model B_alg = 4N + 8Nc + n_R (24 Nc + (16 + 4 norm) N): filter 16Nc,
+    c2r 8Nc + 4N, f_coll box sum 4N, threshold 12N.  A fused kernel carries the
+    model bytes of the stages it replaces: columns = filter; lines = c2r plus
+    the box sum (nonlinear input) or plus the threshold (linear input); rows
+    are the middle of the c2r and carry none of their own."""
     N = D ** 3
     Nc = D * D * (D // 2 + 1)
     return {
@@ -39,9 +43,9 @@ def alg_bytes(D):
         "c2r": 8 * Nc + 4 * N,
         "fcoll_sum": 4 * N,
         "threshold": 12 * N,
-        "fused_columns": 16 * Nc,   # filter + first transform axis: read and write the spectrum
-        "fused_rows": 16 * Nc,      # second axis: read and write the spectrum
-        "fused_lines": 8 * Nc + 12 * N,  # last axis c2r + threshold: read spectrum, read/write state, ...
+        "fused_columns": 16 * Nc,
+        "fused_rows": 0,
+        "fused_lines": 8 * Nc + 4 * N + (4 * N if norm else 12 * N),
     }
 
 
@@ -90,6 +94,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--generic", action="store_true", help="rocFFT c2r + separate kernels instead of the fused path")
+    ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -132,16 +137,21 @@ def main():
     prm = lib.BubbleParams(box_len=L, deltac=float(np.float32(deltac)), zeta=zeta,
                            growth_inv=float(np.float32(deltac / 1.686)), fc_mean=float(fc_mean), linear_input=0,
                            filter_type=1, ref_fallthrough=1, update=0, sphere_periodic=0, sphere_r_scale=1.0,
-                           fused=0 if args.generic else 1)
+                           fused=0 if args.generic else 1, graph=0 if args.no_graph else 1)
     plan = lib.plan(D, 1)
     nbytes = lib.query("cm21_find_bubbles_workspace_bytes", D, len(radii), 0)
     work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
     ion = torch.empty_like(upd)
     r_arr = np.ascontiguousarray(radii, np.float32)
     d_arr = np.ascontiguousarray(denoms, np.float32)
-    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()  # graph capture needs a non-default stream
 
     def step():
+        with torch.cuda.stream(stream):
+            _step()
+
+    def _step():
         lib.call("cm21_find_bubbles", plan.handle, lib.ptr(upd), lib.ptr(ion), r_arr.ctypes.data_as(lib.c_float_p),
                  d_arr.ctypes.data_as(lib.c_float_p), len(radii), prm, lib.ptr(work), nbytes,
                  lib.stream_handle(stream))
@@ -160,26 +170,28 @@ def main():
     dom = max(calib, key=lambda k: calib[k][0])
     stages_all = {k: {"ms_per_launch": ms / n, "launches_per_box": n / 2} for k, (ms, n) in calib.items()}
 
-    # the same steps with no events at all, to show what the instrumentation costs
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    ms_no_events = (time.perf_counter() - t0) * 1e3 / args.steps
-
-    # timed region: only the dominant stage records HIP events
-    lib.profile_enable([dom])
+    # timed region 1 (the reported value): K steps, no instrumentation; with
+    # graph replay unless --no-graph
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    lib.profile_enable(False)
-    prof = lib.profile_collect()
     ms_step = elapsed * 1e3 / args.steps
     boxes_s = args.steps / elapsed
+
+    # timed region 2 (the roofline): the same K steps launched eagerly with HIP
+    # events around every launch of the dominant stage, on the launch stream
+    lib.profile_enable([dom])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms_step_events = (time.perf_counter() - t0) * 1e3 / args.steps
+    lib.profile_enable(False)
+    prof = lib.profile_collect()
     x_HI = 1.0 - float(ion.double().mean())
     ab = alg_bytes(D)
     for k, v in stages_all.items():
@@ -207,7 +219,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_step,
-        "ms_per_step_without_events": ms_no_events,
+        "ms_per_step_eager_with_events": ms_step_events,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -217,7 +229,7 @@ def main():
                                "Zel'dovich move (BOX_LEN=300 Mpc, 40 radii, zeta=30, HII_FILTER=1)",
                    "HII_DIM": D, "DIM": DIM, "BOX_LEN": L, "n_radii": len(radii), "zeta": zeta,
                    "sigma_min2": smin, "growth": growth,
-                   "path": "plain-generic" if args.generic else "plain-fused"},
+                   "path": ("plain-generic" if args.generic else "plain-fused") + ("" if args.no_graph else "+graph")},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": ab[dom], "ms_per_launch": dom_ms},

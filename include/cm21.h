@@ -128,6 +128,10 @@ typedef struct cm21_bubble_params {
     int fused;            /* 1 = fused pruned transforms where supported (update 0,
                              filter_type 1 or 2, D in {64,96,128,160,256,512});
                              0 = rocFFT c2r + separate filter/threshold kernels */
+    int graph;            /* 1 = capture the loop into a HIP graph on the first call
+                             and replay it for identical later calls (same
+                             pointers, parameters, radii, denominators and a
+                             non-null stream); ignored while profiling */
 } cm21_bubble_params;
 
 size_t cm21_find_bubbles_workspace_bytes(int D, int n_radii, int update);

@@ -214,3 +214,33 @@ def test_threshold_near_band_constant_box(gpu, offset, LE):
         assert np.array_equal(got == 1, want == 1)
     part = (want < 1) & (got < 1)
     assert np.allclose(got[part], want[part], rtol=1e-4)
+
+
+@pytest.mark.parametrize("LE", [True, False])
+def test_graph_replay_matches_eager(gpu, LE):
+    # The captured loop replayed several times, with unrelated eager work and
+    # synchronisations between replays, gives the eager result bit for bit.
+    fb = pkg("Programs.find_HII_bubbles")
+    g = pkg("gpu")
+    sig = pkg("cosmo_files.sigmas")
+    D, L = 128, 200.0
+    sp = sig.PowerLawSpectrum(D, L)
+    delta = g.c2r(g.grf_init(D, L, sp.amp, 8))
+    if not LE:
+        delta = (delta * 0.5).contiguous()
+    kw = dict(LE=LE, quiet=True, deltac=1.686 if LE else 1.686 / 0.5, sigma_min2=0.5 if LE else 2.5, zeta=30.0,
+              sig0=sp.sig0_discrete)
+    ref = fb.conv_bubbles(delta, {"BoxSize": L, "HIIdim": D}, **kw)
+    side = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    other = g.c2r(g.grf_init(64, 100.0, 1.0, 1))
+    for rep in range(4):
+        with torch.cuda.stream(side):
+            got = fb.conv_bubbles(delta, {"BoxSize": L, "HIIdim": D}, graph=True, **kw)
+        side.synchronize()
+        assert torch.equal(got, ref), rep
+        # eager work of the program between replays
+        other = other * 1.0001 + 0.5
+        fb.conv_bubbles(other, {"BoxSize": 100.0, "HIIdim": 64}, LE=True, quiet=True, deltac=1.686, sigma_min2=0.5,
+                        zeta=20.0)
+        torch.cuda.synchronize()
