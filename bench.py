@@ -86,6 +86,23 @@ def cpu_baseline(delta, L, radii, denoms, zeta, deltac, fc_mean, budget_s=15.0):
                       f"extrapolated to one box"}
 
 
+def loop_only(args, lib, torch):
+    """The radius loop alone on a GRF box (D=256, L=300) scaled by the growth
+    0.12 as nonlinear input: a short program for counter collection."""
+    g = importlib.import_module("21cmpy_amd.gpu")
+    fb = importlib.import_module("21cmpy_amd.Programs.find_HII_bubbles")
+    sig = importlib.import_module("21cmpy_amd.cosmo_files.sigmas")
+    D, L, growth = 256, 300.0, 0.12
+    sp = sig.PowerLawSpectrum(D, L)
+    delta = (g.c2r(g.grf_init(D, L, sp.amp, 0)) * growth).contiguous()
+    kw = dict(LE=False, quiet=True, deltac=1.686 / growth, sigma_min2=4.0, zeta=30.0, sig0=sp.sig0_discrete,
+              fused=not args.generic)
+    for _ in range(max(args.steps, 1)):
+        fb.conv_bubbles(delta, {"BoxSize": L, "HIIdim": D}, **kw)
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": "loop", "D": D, "steps": args.steps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +112,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--generic", action="store_true", help="rocFFT c2r + separate kernels instead of the fused path")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a HIP graph")
+    ap.add_argument("--mode", choices=["bench", "loop", "copy"], default="bench",
+                    help="loop: only the radius loop on a GRF box (for counter runs); copy: a 512 MiB device "
+                         "copy (to calibrate FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -105,7 +125,18 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.mode == "copy":
+        n = 512 * 1024 * 1024 // 4
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(max(args.steps, 1)):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "copy", "bytes_per_launch_read": 4 * n, "bytes_per_launch_write": 4 * n}))
+        return
     lib = importlib.import_module("21cmpy_amd._lib")
+    if args.mode == "loop":
+        return loop_only(args, lib, torch)
     init = importlib.import_module("21cmpy_amd.Programs.initialize")
     pert = importlib.import_module("21cmpy_amd.Programs.perturb_field")
     fb = importlib.import_module("21cmpy_amd.Programs.find_HII_bubbles")
@@ -198,10 +229,12 @@ def main():
         v["gbs_alg"] = (ab[k] / (v["ms_per_launch"] * 1e-3) / 1e9) if k in ab else None
     dom_ms = prof[dom][0] / prof[dom][1]
     achieved = ab[dom] / (dom_ms * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_D{D}.json")
-    if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
+    traffic = None  # calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, when a summary exists
+    tfile = os.path.join(ROOT, "profiles", f"{dom}_D{D}_traffic.txt")
+    if os.path.exists(tfile) and not args.generic:
+        for line in open(tfile):
+            if line.startswith("bytes_per_launch:"):
+                traffic = float(line.split(":", 1)[1])
 
     # delta_T and P21(k) in 40 log bins k_fund..k_nyq on the result (once)
     def post():
