@@ -20,7 +20,8 @@ SOURCES = [
     "cm21_deltat.hip",
 ]
 
-# -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 (build flags).
+# -packed-fp32-ops off: the compiler emits no packed FP32 add/mul/fma, whose
+#   results have been unreliable on this hardware.
 # -ffp-contract=off: the parity-critical float32 sequences must not be fused.
 # -munsafe-fp-atomics: hardware float atomics on coarse-grained device memory.
 HIPCC_FLAGS = [
