@@ -48,6 +48,35 @@ class BubbleParams(ctypes.Structure):
     ]
 
 
+class BatchBox(ctypes.Structure):
+    """cm21_batch_box (include/cm21.h)."""
+    _fields_ = [
+        ("deltac", ctypes.c_float),
+        ("growth_inv", ctypes.c_float),
+        ("zeta", ctypes.c_double),
+        ("fc_mean", ctypes.c_float),
+        ("reserved", ctypes.c_int),
+    ]
+
+
+class BatchGroup(ctypes.Structure):
+    """cm21_batch_group (include/cm21.h)."""
+    _fields_ = [
+        ("D", ctypes.c_int),
+        ("nbox", ctypes.c_int),
+        ("box_len", ctypes.c_float),
+        ("n_radii", ctypes.c_int),
+        ("radii", ctypes.POINTER(ctypes.c_float)),
+        ("denoms", ctypes.POINTER(ctypes.c_float)),
+        ("boxes", ctypes.POINTER(BatchBox)),
+        ("delta", ctypes.c_void_p),
+        ("ionized", ctypes.c_void_p),
+        ("plan", ctypes.c_void_p),
+        ("tile_columns", ctypes.c_int),
+        ("tile_lines", ctypes.c_int),
+    ]
+
+
 VP, I, F, D, SZ, U32 = (ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double,
                          ctypes.c_size_t, ctypes.c_uint32)
 
@@ -76,6 +105,13 @@ SIGNATURES = {
     "cm21_power_spectrum": (I, [VP, VP, VP, I, D, VP, I, I, VP, VP, VP, VP, SZ, VP]),
     "cm21_profile_enable": (I, [I]),
     "cm21_profile_collect": (I, [c_double_p, ctypes.POINTER(ctypes.c_int64), I]),
+    "cm21_find_bubbles_batched_workspace_bytes": (SZ, [ctypes.POINTER(BatchGroup), I,
+                                                       ctypes.POINTER(BubbleParams)]),
+    "cm21_find_bubbles_batched": (I, [ctypes.POINTER(BatchGroup), I, ctypes.POINTER(BubbleParams), VP, SZ, VP]),
+    "cm21_batch_tile_count": (I, [I, I]),
+    "cm21_batch_tile_name": (ctypes.c_char_p, [I, I, I]),
+    "cm21_batch_autotune": (I, [ctypes.POINTER(BatchGroup), I, ctypes.POINTER(BubbleParams), VP, SZ, I, c_double_p,
+                                VP]),
 }
 
 STAGES = ["r2c", "filter", "c2r", "fcoll_sum", "threshold", "sphere", "fused_columns", "fused_rows",

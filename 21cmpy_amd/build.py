@@ -15,6 +15,7 @@ SOURCES = [
     "cm21_core.hip",
     "cm21_bubbles.hip",
     "cm21_fused.hip",
+    "cm21_batched.hip",
     "cm21_init.hip",
     "cm21_perturb.hip",
     "cm21_deltat.hip",

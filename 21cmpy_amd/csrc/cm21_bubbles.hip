@@ -17,12 +17,6 @@
 
 #include "cm21_internal.hpp"
 #include "cm21_window.hpp"
-
-namespace cm21 {
-int plan_dim(const cm21_plan* p);
-int plan_batch(const cm21_plan* p);
-const float2* plan_twiddles(const cm21_plan* p);
-}
 #include "cm21_fused_api.hpp"
 
 using namespace cm21;

@@ -60,6 +60,7 @@ extern "C" const char* cm21_last_error(void) { return g_last_error.c_str(); }
 static void plan_release(cm21_plan* p) {
     if (!p) return;
     cm21::forget_plan_graphs(p);
+    cm21::forget_batch_graphs();
     if (p->fwd) rocfft_plan_destroy(p->fwd);
     if (p->inv) rocfft_plan_destroy(p->inv);
     if (p->info) rocfft_execution_info_destroy(p->info);

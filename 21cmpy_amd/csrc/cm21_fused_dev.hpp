@@ -1,0 +1,262 @@
+// Device bodies of the fused, pruned inverse transforms, shared by the
+// single-box radius loop (cm21_fused.hip) and the batched loop
+// (cm21_batched.hip) so that both run the same arithmetic.
+//
+// Layout conventions: S is a c2r-ready half spectrum S[a][b][c] (c fastest,
+// H = D/2+1 entries).  The columns and rows passes write T[p][q][c] with a row
+// stride `ho` >= cmax+1 (H for the single-box loop, cmax+1 for the compact
+// per-radius storage of the batched loop).  The lines pass reads the prefix
+// c < cl of each line and produces the D real values of the line.
+#pragma once
+
+#include <cmath>
+
+#include "cm21_dft.hpp"
+#include "cm21_fused_api.hpp"
+#include "cm21_internal.hpp"
+#include "cm21_window.hpp"
+
+// The transforms may contract a*b+c into FMAs; the parity-critical float32
+// steps use explicit __f*_rn intrinsics, which are never contracted.
+#pragma clang fp contract(fast)
+
+namespace cm21 {
+
+enum LineMode { kLineSum = 0, kLineThreshold = 1, kLineFinal = 2 };
+
+// Window of one mode from its integer |n|^2.  Inside the shell m2 >= m2lo the
+// reference's float32 cut test decides (find_bubbles.cu:22-35); the Gaussian
+// factor is exp2(m2 * gcoef); `scale` folds 1/N (and the growth factor of
+// nonlinear input).
+__device__ inline float2 fused_window(float2 x, int a, int b, int c, int m2, int D, const FusedParams& fp,
+                                      float scale) {
+    if (fp.exact_cut && m2 >= fp.m2lo) {
+        const float kR = __fmul_rn(kmag_of(a, b, c, D, fp.dk, 0), fp.R);
+        if ((float)((double)kR * 0.413566994) > 1.0f) return make_float2(0.0f, 0.0f);
+    }
+    if (fp.gcoef != 0.0f) {
+        const float w2 = exp2f((float)m2 * fp.gcoef);
+        x = make_float2(__fmul_rn(x.x, w2), __fmul_rn(x.y, w2));
+    }
+    return make_float2(__fmul_rn(x.x, scale), __fmul_rn(x.y, scale));
+}
+
+// Columns pass of one block: window + inverse DFT along axis 0 for the W
+// columns (b, c0..c0+W-1), b given by its slot among the surviving rows
+// (0..bpos positive, then the negative modes).  lds: W*(D+1) float2.
+template <int D1, int D2, int W>
+__device__ __forceinline__ void columns_body(const float2* __restrict__ S, float2* __restrict__ T, int ho,
+                                             const float2* __restrict__ tw, const FusedParams& fp, float scale,
+                                             int slot, int cgroup, float2* lds) {
+    constexpr int D = D1 * D2, H = D / 2 + 1, NT = W * D1, LD = D + 1;
+    const int t = threadIdx.x;
+    const int j = t % W, n1 = t / W;
+    const int bpos = fp.bm < D / 2 ? fp.bm : D / 2;
+    const int b = slot <= bpos ? slot : D - (slot - bpos);
+    const int bs = mode_of(b, D);
+    const int c0 = cgroup * W;
+    if (bs * bs + c0 * c0 > fp.m2max) return;  // whole group outside the disc (uniform)
+    const int c = c0 + j;
+    const bool col_ok = c < H && bs * bs + c * c <= fp.m2max;
+    float2 v[D2];
+#pragma unroll
+    for (int n2 = 0; n2 < D2; ++n2) {
+        const int a = n1 + D1 * n2;
+        const int as = mode_of(a, D);
+        const int m2 = as * as + bs * bs + c * c;
+        float2 x = make_float2(0.0f, 0.0f);
+        if (col_ok && m2 <= fp.m2max) x = fused_window(S[((size_t)a * D + b) * H + c], a, b, c, m2, D, fp, scale);
+        v[n2] = x;
+    }
+    dft<D2>(v);
+#pragma unroll
+    for (int k2 = 0; k2 < D2; ++k2) lds[j * LD + k2 * D1 + n1] = (n1 * k2 == 0) ? v[k2] : cmul(v[k2], tw[n1 * k2]);
+    __syncthreads();
+    for (int it = t; it < W * D2; it += NT) {
+        const int jj = it % W, k2 = it / W;
+        const int cc = c0 + jj;
+        float2 u[D1];
+#pragma unroll
+        for (int n = 0; n < D1; ++n) u[n] = lds[jj * LD + k2 * D1 + n];
+        dft<D1>(u);
+        if (cc < H && bs * bs + cc * cc <= fp.m2max) {
+#pragma unroll
+            for (int k1 = 0; k1 < D1; ++k1) T[((size_t)(k2 + D2 * k1) * D + b) * ho + cc] = u[k1];
+        }
+    }
+}
+
+// Rows pass of one block: inverse DFT along axis 1, in place, for the W
+// columns (p, c0..c0+W-1).  lds: W*(D+1) float2.
+template <int D1, int D2, int W>
+__device__ __forceinline__ void rows_body(float2* __restrict__ T, int ho, const float2* __restrict__ tw,
+                                          const FusedParams& fp, int p, int cgroup, float2* lds) {
+    constexpr int D = D1 * D2, H = D / 2 + 1, NT = W * D1, LD = D + 1;
+    const int t = threadIdx.x;
+    const int j = t % W, n1 = t / W;
+    const int c0 = cgroup * W;
+    const int c = c0 + j;
+    float2 v[D2];
+#pragma unroll
+    for (int n2 = 0; n2 < D2; ++n2) {
+        const int b = n1 + D1 * n2;
+        const int bs = mode_of(b, D);
+        v[n2] = (c < H && bs * bs + c * c <= fp.m2max) ? T[((size_t)p * D + b) * ho + c] : make_float2(0.0f, 0.0f);
+    }
+    dft<D2>(v);
+#pragma unroll
+    for (int k2 = 0; k2 < D2; ++k2) lds[j * LD + k2 * D1 + n1] = (n1 * k2 == 0) ? v[k2] : cmul(v[k2], tw[n1 * k2]);
+    __syncthreads();
+    for (int it = t; it < W * D2; it += NT) {
+        const int jj = it % W, k2 = it / W;
+        const int cc = c0 + jj;
+        float2 u[D1];
+#pragma unroll
+        for (int n = 0; n < D1; ++n) u[n] = lds[jj * LD + k2 * D1 + n];
+        dft<D1>(u);
+        if (cc <= fp.cmax) {
+#pragma unroll
+            for (int k1 = 0; k1 < D1; ++k1) T[((size_t)p * D + (k2 + D2 * k1)) * ho + cc] = u[k1];
+        }
+    }
+}
+
+// c2r of L consecutive lines along the last axis.  The length-D real output
+// is the length-N = D/2 complex transform of
+// Z[k] = (X[k] + conj X[N-k]) + i (X[k] - conj X[N-k]) w_D^k, whose outputs
+// hold x[2m] (real part) and x[2m+1] (imaginary part).  One LDS buffer
+// (L*HP float2) holds, in turn, the half lines, the transposed partial
+// transforms and the real outputs; rows are padded to an odd number of
+// float2 so that lanes walking different lines hit different banks.
+template <int N1, int N2, int L>
+struct LinesCore {
+    static constexpr int N = N1 * N2, D = 2 * N, H = N + 1, NT = L * N1;
+    static constexpr int HP = H | 1;
+    static constexpr int LOADS = (L * H + NT - 1) / NT;
+    static constexpr int ITEMS = (L * N2 + NT - 1) / NT;  // step-3 transforms per thread
+    static constexpr int WORDS = L * D / 64;              // mask words of the block's cells
+    static_assert(NT % 64 == 0 && (L * D) % 64 == 0, "waves must cover whole mask words");
+    static_assert(L * D / 64 <= NT, "one thread per mask word");
+
+    // Global loads of the prefixes c < cl of L lines starting at U (line stride hs).
+    __device__ static __forceinline__ void load(const float2* __restrict__ U, int hs, int cl, float2 (&tmp)[LOADS]) {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < LOADS; ++q) {
+            const int f = t + q * NT;
+            const int i = f / H, c = f - i * H;
+            tmp[q] = (f < L * H && c < cl) ? U[(size_t)i * hs + c] : make_float2(0.0f, 0.0f);
+        }
+    }
+
+    __device__ static __forceinline__ void store(const float2 (&tmp)[LOADS], float2* buf) {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < LOADS; ++q) {
+            const int f = t + q * NT;
+            const int i = f / H, c = f - i * H;
+            if (f < L * H) buf[i * HP + c] = tmp[q];
+        }
+    }
+
+    // In: buf holds the half lines (after a barrier).  Out: buf holds the real
+    // lines, line i at ((float*)buf)[i * 2 * HP + r], followed by a barrier.
+    __device__ static __forceinline__ void transform(float2* buf, const float2* stw) {
+        const int t = threadIdx.x;
+        {
+            const int i = t % L, n1 = t / L;  // lanes walk lines: distinct banks
+            float2 v[N2];
+#pragma unroll
+            for (int n2 = 0; n2 < N2; ++n2) {
+                const int k = n1 + N1 * n2;
+                const float2 X = buf[i * HP + k];
+                const float2 Xm = cconj(buf[i * HP + (N - k)]);
+                const float2 s = cadd(X, Xm);
+                const float2 d = csub(X, Xm);
+                v[n2] = cadd(s, times_i(k == 0 ? d : cmul(d, stw[k])));
+            }
+            dft<N2>(v);
+            __syncthreads();
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2)
+                buf[i * HP + k2 * N1 + n1] = (n1 * k2 == 0) ? v[k2] : cmul(v[k2], stw[2 * n1 * k2]);
+        }
+        __syncthreads();
+        float2 u[ITEMS][N1];
+#pragma unroll
+        for (int q = 0; q < ITEMS; ++q) {
+            const int it = t + q * NT;
+            const int ii = it % L, k2 = it / L;  // lanes walk lines: distinct banks
+            if (it < L * N2) {
+#pragma unroll
+                for (int n = 0; n < N1; ++n) u[q][n] = buf[ii * HP + k2 * N1 + n];
+                dft<N1>(u[q]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < ITEMS; ++q) {
+            const int it = t + q * NT;
+            const int ii = it % L, k2 = it / L;
+            if (it < L * N2) {
+#pragma unroll
+                for (int k1 = 0; k1 < N1; ++k1) buf[ii * HP + k2 + N2 * k1] = u[q][k1];
+            }
+        }
+        __syncthreads();
+    }
+};
+
+__device__ inline float fcoll_of(float smoothed, float deltac, float denom) {
+    return erfcf(__fdiv_rn(__fsub_rn(deltac, smoothed), denom));
+}
+
+// Threshold of the centre-pixel rule in density space.  f = erfcf((deltac -
+// d)/denom) * scale >= 1/zeta holds exactly when d >= d*, with
+// d* = deltac - denom * erfcinv(1/(zeta*scale)), up to the float32 rounding of
+// the left side; cells within `band` of d* take the exact float32 test, the
+// others are decided by comparing d with d*.
+struct DensityThreshold {
+    float dstar;  // decided by d > dstar outside the band
+    float band;   // +inf: every cell takes the exact test
+};
+
+__device__ inline DensityThreshold density_threshold(float deltac, float denom, double zeta, float scale) {
+    const double y = 1.0 / (zeta * (double)scale);
+    DensityThreshold t;
+    if (!(y > 1e-30 && y < 1.999)) {
+        // degenerate targets (a zero box mean, erfc's end points): every cell
+        // takes the exact float32 test
+        t.dstar = 0.0f;
+        t.band = INFINITY;
+        return t;
+    }
+    const double ds = (double)deltac - (double)denom * erfcinv(y);
+    t.dstar = (float)ds;
+    t.band = (float)(1e-4 * (1.0 + fabs((double)deltac) + (double)denom + fabs(ds)));
+    return t;
+}
+
+// The centre-pixel decision of one cell (find_bubbles.cu:53-70): zeta*f >= 1,
+// tested as (double)f >= 1/zeta with f = erfcf(...) (times the box-mean
+// scale for nonlinear input).
+__device__ __forceinline__ bool cell_hit(float delta, const DensityThreshold& th, float deltac, float denom,
+                                         bool use_scale, float scale, double inv_zeta) {
+    if (fabsf(delta - th.dstar) <= th.band) {
+        float fc = fcoll_of(delta, deltac, denom);
+        if (use_scale) fc = __fmul_rn(fc, scale);
+        return (double)fc >= inv_zeta;
+    }
+    return delta > th.dstar;
+}
+
+// Final-radius value of a cell not yet ionized (find_bubbles.cu:142-158):
+// 1 when zeta*f >= 1, else the partial zeta*f.
+__device__ __forceinline__ float final_value(float delta, float deltac, float denom, bool use_scale, float scale,
+                                             double zeta, double inv_zeta) {
+    float fc = fcoll_of(delta, deltac, denom);
+    if (use_scale) fc = __fmul_rn(fc, scale);
+    return ((double)fc >= inv_zeta) ? 1.0f : (float)((double)fc * zeta);
+}
+
+}  // namespace cm21

@@ -62,7 +62,14 @@ ProfToken prof_begin(int stage, hipStream_t st);
 void prof_end(const ProfToken& t, hipStream_t st);
 bool prof_any();  // any stage instrumented
 
+// Plan accessors (cm21_core.hip).
+int plan_dim(const ::cm21_plan* p);
+int plan_batch(const ::cm21_plan* p);
+const float2* plan_twiddles(const ::cm21_plan* p);
+
+// Drop captured graphs that reference a plan about to be destroyed.
 void forget_plan_graphs(const ::cm21_plan* plan);
+void forget_batch_graphs();
 
 // Signed mode number of FFT index i on an axis of length D; index D/2 maps
 // to +D/2 (Programs/find_bubbles.cu:22-24).

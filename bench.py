@@ -103,6 +103,61 @@ def loop_only(args, lib, torch):
     print(json.dumps({"mode": "loop", "D": D, "steps": args.steps}))
 
 
+def batched_bench(args, torch, tune=True):
+    """The 64-box batch (mixed HII_DIM 96/128/160, per-box zeta, z, sigma_min^2
+    from the committed manifest; linear z=0 GRF boxes, cell 1.5625 Mpc, 37
+    radii, HII_FILTER 1) through cm21_find_bubbles_batched as one graph.
+    Returns the result object for the JSON line."""
+    bm = importlib.import_module("21cmpy_amd.Programs.find_HII_bubbles_batched")
+    g = importlib.import_module("21cmpy_amd.gpu")
+    sig = importlib.import_module("21cmpy_amd.cosmo_files.sigmas")
+    lib = importlib.import_module("21cmpy_amd._lib")
+    specs = bm.load_manifest()["boxes"]
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        bb = bm.BatchedBubbles(specs, fil=1, LE=True, graph=not args.no_graph, stream=stream)
+        amps = {}
+        for i, s in enumerate(specs):
+            key = (s["D"], s["box_len"])
+            if key not in amps:
+                amps[key] = sig.PowerLawSpectrum(*key).amp
+            bb.set_delta(i, g.c2r(g.grf_init(s["D"], s["box_len"], amps[key], s["seed"])))
+        tuned = bb.autotune(reps=3) if tune else None
+    torch.cuda.synchronize()
+    for _ in range(max(args.warmup, 1)):
+        bb.run()
+    torch.cuda.synchronize()
+    lib.profile_collect()
+    lib.profile_enable(True)
+    bb.run()
+    torch.cuda.synchronize()
+    lib.profile_enable(False)
+    stages = {k: {"ms": ms, "launches": n} for k, (ms, n) in lib.profile_collect().items()}
+    steps = max(args.steps, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bb.run()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    nb = len(specs)
+    value = nb * steps / elapsed
+    fb = importlib.import_module("21cmpy_amd.Programs.find_HII_bubbles")
+    balg = sum(b_alg(s["D"], len(fb.radius_schedule(s["box_len"], s["D"])[0]), 0) for s in specs)
+    xhi = bb.x_HI()
+    return {
+        "value": value, "unit": "boxes/s", "boxes_per_batch": nb, "steps": steps,
+        "ms_per_batch": elapsed * 1e3 / steps,
+        "workload": "64 linear GRF boxes, HII_DIM in {96,128,160} (21/25/18), BOX_LEN = 1.5625 Mpc x HII_DIM, "
+                    "37 radii, per-box zeta in [10,60], z in {7..10} (deltac = 1.686/D(z), D(z)=0.12*9/(1+z)), "
+                    "sigma_min^2 in [2,6], HII_FILTER 1; one graph per batch",
+        "model_roofline": {"B_alg_per_batch": balg, "frac": balg * steps / elapsed / (HBM_PEAK_GBS * 1e9)},
+        "stages_one_batch_eager": stages,
+        "tiles": {k: v["chosen"] for k, v in (tuned or {}).items()},
+        "x_HI_mean": float(np.mean(xhi)), "x_HI_min": float(np.min(xhi)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,7 +167,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--generic", action="store_true", help="rocFFT c2r + separate kernels instead of the fused path")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a HIP graph")
-    ap.add_argument("--mode", choices=["bench", "loop", "copy"], default="bench",
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-box batched measurement")
+    ap.add_argument("--mode", choices=["bench", "loop", "copy", "batched"], default="bench",
                     help="loop: only the radius loop on a GRF box (for counter runs); copy: a 512 MiB device "
                          "copy (to calibrate FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -137,6 +193,9 @@ def main():
     lib = importlib.import_module("21cmpy_amd._lib")
     if args.mode == "loop":
         return loop_only(args, lib, torch)
+    if args.mode == "batched":
+        print(json.dumps({"mode": "batched", "batched": batched_bench(args, torch)}))
+        return
     init = importlib.import_module("21cmpy_amd.Programs.initialize")
     pert = importlib.import_module("21cmpy_amd.Programs.perturb_field")
     fb = importlib.import_module("21cmpy_amd.Programs.find_HII_bubbles")
@@ -272,6 +331,11 @@ def main():
         "pipeline_ms": {"initialize_512_to_256": t_init, "perturb_cic": t_pert, "delta_T_and_P21": t_post},
         "x_HI": x_HI,
     }
+    if not args.no_batched:
+        del work, ion
+        torch.cuda.empty_cache()
+        out["batched"] = batched_bench(args, torch)
+        out["batched"]["vs_plain"] = out["batched"]["value"] / boxes_s
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(upd.cpu().numpy(), L, radii, denoms, zeta, deltac, fc_mean,
                                            args.cpu_budget)

@@ -215,6 +215,59 @@ int cm21_power_spectrum(cm21_plan* plan, const float* field,
 int cm21_profile_enable(int stage_mask);
 int cm21_profile_collect(double* ms, int64_t* count, int nstages);
 
+/* ---- Batched find_HII_bubbles.  No reference counterpart: the reference
+ * runs one box per conv_bubbles call (Programs/find_HII_bubbles.py:17-193,
+ * driven box by box from Programs/Drives.py:7-14).  Boxes are grouped by side
+ * D; every box of a group shares D, box_len and the radius list, and has its
+ * own barrier, efficiency and denominators.  The per-cell arithmetic is that
+ * of cm21_find_bubbles with params->fused = 1 (centre-pixel update, filter 1
+ * or 2). */
+typedef struct cm21_batch_box {
+    float deltac;     /* barrier of this box (1.686 / D(z) for linear z=0 input) */
+    float growth_inv; /* nonlinear input: deltac / 1.686 */
+    double zeta;      /* ionizing efficiency */
+    float fc_mean;    /* nonlinear input: mean collapsed fraction */
+    int reserved;
+} cm21_batch_box;
+
+typedef struct cm21_batch_group {
+    int D;                       /* 64, 96, 128, 160, 256 or 512 */
+    int nbox;                    /* boxes in the group */
+    float box_len;               /* Mpc, shared */
+    int n_radii;                 /* shared radius list, host float32 */
+    const float* radii;
+    const float* denoms;         /* host, nbox x n_radii float32 */
+    const cm21_batch_box* boxes; /* host, nbox */
+    const float* delta;          /* device, nbox boxes of D^3 float32, contiguous */
+    float* ionized;              /* device, nbox boxes */
+    cm21_plan* plan;             /* cm21_plan_create(&p, D, nbox) */
+    int tile_columns;            /* candidate index, -1 = default */
+    int tile_lines;              /* candidate index, -1 = default */
+} cm21_batch_group;
+
+/* params: filter_type, ref_fallthrough, linear_input, update (must be 0) and
+ * graph are used; the per-box fields come from the groups.  With graph = 1
+ * and a non-null stream the launches after the table upload are captured
+ * into a HIP graph on the first call and replayed for identical calls.
+ * Returns 0 from the size query for invalid groups. */
+size_t cm21_find_bubbles_batched_workspace_bytes(const cm21_batch_group* groups, int ngroups,
+                                                 const cm21_bubble_params* params);
+int cm21_find_bubbles_batched(const cm21_batch_group* groups, int ngroups, const cm21_bubble_params* params,
+                              void* work, size_t work_bytes, void* stream);
+
+/* Tile candidates per D: stage 0 = columns/rows passes, 1 = lines pass.
+ * cm21_batch_tile_count returns -1 for an unsupported D. */
+int cm21_batch_tile_count(int D, int stage);
+const char* cm21_batch_tile_name(int D, int stage, int index);
+
+/* Times every candidate of both stages on the caller's data (reps timed runs
+ * each, after one warm run) and writes the fastest into each group's
+ * tile_columns / tile_lines.  ms_out (optional) receives the mean time of
+ * each candidate, group by group, columns candidates first.  Leaves the
+ * outputs unspecified; synchronises the stream. */
+int cm21_batch_autotune(cm21_batch_group* groups, int ngroups, const cm21_bubble_params* params, void* work,
+                        size_t work_bytes, int reps, double* ms_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,11 +81,13 @@ def test_sphere_paint_matches_oracle(gpu, periodic):
     g = pkg("gpu")
     D = 24
     rng = np.random.default_rng(7)
-    sm = (rng.standard_normal((D, D, D)) * 0.5).astype(np.float32)
-    deltac, denom, zeta = 1.686, np.float32(1.0), 30.0
+    sm = (rng.standard_normal((D, D, D)) * 0.45).astype(np.float32)
+    # zeta = 1.6: centres where delta >~ 1.34, about 3 sigma (a few dozen cells)
+    deltac, denom, zeta = 1.686, np.float32(1.0), 1.6
     fc = bubbles.fcoll_f32(sm, deltac, denom)
     centers = fc.astype(np.float64) >= 1 / zeta
-    assert 0 < centers.sum() < 60
+    assert 0 < centers.sum() < 80
+    assert np.min(np.abs(fc.astype(np.float64) * zeta - 1.0)) > 1e-6  # no cell on the threshold
     for R in (1.5, 3.7, 9.2):
         ion = torch.zeros((D, D, D), device="cuda")
         g.update_sphere(ion, _dev(sm), deltac, denom, zeta, R, periodic)

@@ -35,7 +35,7 @@ def cdll():
 def test_header_parse_finds_all_entry_points():
     names = header_functions()
     assert "cm21_find_bubbles" in names and "cm21_power_spectrum" in names
-    assert "cm21_delta_T" in names and len(names) == 23
+    assert "cm21_delta_T" in names and len(names) == 28
 
 
 def test_library_exports_every_declared_symbol(cdll):
@@ -206,3 +206,35 @@ def test_init_params_values():
     assert (P.R_BUBBLE_MAX, P.R_BUBBLE_MIN, P.DELTA_R_FACTOR) == (30.0, 0.5, 1.1)
     assert P.ZETA == 40.0 and P.HII_FILTER == 1 and P.FIND_BUBBLE_ALGORITHM == 0
     assert P.L_FACTOR == np.float32(0.620350491) and P.ION_Tvir_MIN == 1.0e4
+
+
+def test_batched_workspace_query(cdll):
+    """The batched size query is host arithmetic: it accepts the 64-box
+    manifest's groups and rejects unsupported shapes with 0."""
+    import json
+    lib_mod = pkg("_lib")
+    fb = pkg("Programs.find_HII_bubbles")
+    man = json.load(open(os.path.join(ROOT, "21cmpy_amd", "Parameter_files", "batch64_manifest.json")))
+    groups = {}
+    for b in man["boxes"]:
+        groups.setdefault(b["D"], []).append(b)
+    assert sorted(groups) == [96, 128, 160]
+    arr = (lib_mod.BatchGroup * 3)()
+    keep = []
+    for k, D in enumerate(sorted(groups)):
+        radii, _ = fb.radius_schedule(1.5625 * D, D)
+        assert len(radii) == 37
+        keep.append(radii)
+        arr[k] = lib_mod.BatchGroup(D=D, nbox=len(groups[D]), box_len=1.5625 * D, n_radii=len(radii),
+                                    radii=radii.ctypes.data_as(lib_mod.c_float_p), tile_columns=-1, tile_lines=-1)
+    prm = lib_mod.BubbleParams(filter_type=1, ref_fallthrough=1, linear_input=1, update=0, fused=1)
+    n = cdll.cm21_find_bubbles_batched_workspace_bytes(arr, 3, ctypes.byref(prm))
+    spectra = sum(len(groups[D]) * 8 * D * D * (D // 2 + 1) for D in groups)
+    assert spectra < n < 40 * spectra
+    prm.filter_type = 0
+    assert cdll.cm21_find_bubbles_batched_workspace_bytes(arr, 3, ctypes.byref(prm)) == 0
+    prm.filter_type = 1
+    arr[0].D = 100
+    assert cdll.cm21_find_bubbles_batched_workspace_bytes(arr, 3, ctypes.byref(prm)) == 0
+    assert cdll.cm21_batch_tile_count(128, 0) >= 1 and cdll.cm21_batch_tile_count(100, 0) == -1
+    assert cdll.cm21_batch_tile_name(128, 1, 0) is not None
