@@ -103,7 +103,7 @@ def loop_only(args, lib, torch):
     print(json.dumps({"mode": "loop", "D": D, "steps": args.steps}))
 
 
-def batched_bench(args, torch, tune=True):
+def batched_bench(args, torch):
     """The 64-box batch (mixed HII_DIM 96/128/160, per-box zeta, z, sigma_min^2
     from the committed manifest; linear z=0 GRF boxes, cell 1.5625 Mpc, 37
     radii, HII_FILTER 1) through cm21_find_bubbles_batched as one graph.
@@ -122,7 +122,7 @@ def batched_bench(args, torch, tune=True):
             if key not in amps:
                 amps[key] = sig.PowerLawSpectrum(*key).amp
             bb.set_delta(i, g.c2r(g.grf_init(s["D"], s["box_len"], amps[key], s["seed"])))
-        tuned = bb.autotune(reps=3) if tune else None
+        tuned = None if args.no_tune else bb.autotune(reps=3, save_path=args.save_tiles)
     torch.cuda.synchronize()
     for _ in range(max(args.warmup, 1)):
         bb.run()
@@ -153,7 +153,8 @@ def batched_bench(args, torch, tune=True):
                     "sigma_min^2 in [2,6], HII_FILTER 1; one graph per batch",
         "model_roofline": {"B_alg_per_batch": balg, "frac": balg * steps / elapsed / (HBM_PEAK_GBS * 1e9)},
         "stages_one_batch_eager": stages,
-        "tiles": {k: v["chosen"] for k, v in (tuned or {}).items()},
+        "tiles": ({k: v["chosen"] for k, v in tuned.items()} if tuned else
+                  {k: {"columns": c, "lines": l} for k, (c, l) in bb.tiles().items()}),
         "x_HI_mean": float(np.mean(xhi)), "x_HI_min": float(np.min(xhi)),
     }
 
@@ -168,6 +169,8 @@ def main():
     ap.add_argument("--generic", action="store_true", help="rocFFT c2r + separate kernels instead of the fused path")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a HIP graph")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-box batched measurement")
+    ap.add_argument("--no-tune", action="store_true", help="batched: use the stored tile choice, no autotuning")
+    ap.add_argument("--save-tiles", default=None, help="batched: merge the autotuned tiles into this JSON file")
     ap.add_argument("--mode", choices=["bench", "loop", "copy", "batched"], default="bench",
                     help="loop: only the radius loop on a GRF box (for counter runs); copy: a 512 MiB device "
                          "copy (to calibrate FETCH_SIZE/WRITE_SIZE)")
