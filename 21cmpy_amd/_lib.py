@@ -77,6 +77,27 @@ class BatchGroup(ctypes.Structure):
     ]
 
 
+class UnetConv(ctypes.Structure):
+    """cm21_unet_conv (include/cm21.h)."""
+    _fields_ = [
+        ("B", ctypes.c_int), ("D", ctypes.c_int), ("cin", ctypes.c_int), ("cout", ctypes.c_int),
+        ("c0", ctypes.c_int), ("o0", ctypes.c_int),
+        ("x0", ctypes.c_void_p), ("x1", ctypes.c_void_p), ("mask", ctypes.c_void_p), ("w", ctypes.c_void_p),
+        ("bias", ctypes.POINTER(ctypes.c_float)), ("bias_per_sample", ctypes.c_int), ("relu", ctypes.c_int),
+        ("y0", ctypes.c_void_p), ("y1", ctypes.c_void_p),
+    ]
+
+
+class UnetWgrad(ctypes.Structure):
+    """cm21_unet_wgrad (include/cm21.h)."""
+    _fields_ = [
+        ("B", ctypes.c_int), ("D", ctypes.c_int), ("cin", ctypes.c_int), ("cout", ctypes.c_int), ("c0", ctypes.c_int),
+        ("x0", ctypes.c_void_p), ("x1", ctypes.c_void_p), ("g", ctypes.c_void_p), ("mask", ctypes.c_void_p),
+        ("dw", ctypes.POINTER(ctypes.c_float)), ("db", ctypes.POINTER(ctypes.c_float)),
+        ("work", ctypes.c_void_p), ("work_bytes", ctypes.c_size_t),
+    ]
+
+
 VP, I, F, D, SZ, U32 = (ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double,
                          ctypes.c_size_t, ctypes.c_uint32)
 
@@ -112,6 +133,22 @@ SIGNATURES = {
     "cm21_batch_tile_name": (ctypes.c_char_p, [I, I, I]),
     "cm21_batch_autotune": (I, [ctypes.POINTER(BatchGroup), I, ctypes.POINTER(BubbleParams), VP, SZ, I, c_double_p,
                                 VP]),
+    "cm21_unet_conv3_fwd": (I, [ctypes.POINTER(UnetConv), VP]),
+    "cm21_unet_wgrad_workspace_bytes": (SZ, [I, I, I, I]),
+    "cm21_unet_conv3_wgrad": (I, [ctypes.POINTER(UnetWgrad), VP]),
+    "cm21_unet_pool_fwd": (I, [VP, VP, I, I, I, VP]),
+    "cm21_unet_pool_bwd": (I, [VP, VP, I, I, I, I, VP]),
+    "cm21_unet_up_fwd": (I, [VP, c_float_p, c_float_p, VP, I, I, I, I, VP]),
+    "cm21_unet_up_workspace_bytes": (SZ, [I, I, I, I]),
+    "cm21_unet_up_bwd": (I, [VP, VP, c_float_p, VP, c_float_p, c_float_p, I, I, I, I, VP, SZ, VP]),
+    "cm21_unet_in_fwd": (I, [c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, VP, I, I, F, VP]),
+    "cm21_unet_in_workspace_bytes": (SZ, [I, I]),
+    "cm21_unet_in_wgrad": (I, [c_float_p, c_float_p, VP, VP, c_float_p, c_float_p, I, I, F, VP, SZ, VP]),
+    "cm21_unet_head_workspace_bytes": (SZ, []),
+    "cm21_unet_head": (I, [VP, c_float_p, c_float_p, c_float_p, c_float_p, VP, c_float_p, c_double_p, I, I, VP, SZ,
+                           VP]),
+    "cm21_unet_pack": (I, [c_float_p, VP, VP, I, I, VP]),
+    "cm21_adamw": (I, [c_float_p, c_float_p, c_float_p, c_float_p, SZ, F, F, F, F, F, I, VP]),
 }
 
 STAGES = ["r2c", "filter", "c2r", "fcoll_sum", "threshold", "sphere", "fused_columns", "fused_rows",

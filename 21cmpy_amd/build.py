@@ -16,6 +16,7 @@ SOURCES = [
     "cm21_bubbles.hip",
     "cm21_fused.hip",
     "cm21_batched.hip",
+    "cm21_unet.hip",
     "cm21_init.hip",
     "cm21_perturb.hip",
     "cm21_deltat.hip",

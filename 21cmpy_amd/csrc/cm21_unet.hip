@@ -1,0 +1,1027 @@
+// 3D U-Net surrogate of the excursion-set ionization field: the layer
+// kernels of its forward and backward passes and the AdamW update.
+//
+// No counterpart in the reference (Programs/ computes every box with the
+// excursion-set loop); the surrogate is trained on that loop's outputs and
+// checked against it.
+//
+// Tensors are channels-last (B, D, D, D, C) with periodic boundaries (the
+// boxes are periodic), activations in bf16 (uint16 bit patterns, round to
+// nearest even), weights in fp32 masters with bf16 copies for the matrix
+// cores, accumulation in fp32, reductions in a fixed order.
+//
+//   3^3 convolution, forward:  implicit GEMM on v_mfma_f32_16x16x32_bf16.  A
+//     block computes TZ x TY x 16 output voxels for all output channels: the
+//     input tile with a one-voxel halo sits in LDS; for each 32-wide slice of
+//     K = 27 * cin (tap-major, channel-minor) every lane reads 8 consecutive
+//     channels of one voxel (A, 16 bytes from LDS) and 8 consecutive K of one
+//     output channel (B, 16 bytes of the bf16 weights); the 16 x 16 x 32
+//     product accumulates in registers.
+//   3^3 convolution, input gradient: the same kernel on the output gradient
+//     (masked by the ReLU of the forward output) with the weights transposed
+//     and spatially flipped.
+//   3^3 convolution, weight gradient: fp32 VALU over voxel slabs with the
+//     input halo tile and the masked output gradient in LDS; per-block
+//     partials reduced in a fixed order.
+//   2x2x2 average pooling, 2x2x2 stride-2 transposed convolution, the 1^3
+//   head with sigmoid + binary cross entropy, the input layer (density plus
+//   two constant conditioning channels, folded into a per-sample bias), and
+//   AdamW: fp32 VALU.
+#include <cmath>
+#include <cstring>
+
+#include "cm21.h"
+#include "cm21_internal.hpp"
+
+namespace cm21 {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short bf16_t;
+
+__device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((unsigned)h << 16); }
+
+__device__ __forceinline__ bf16_t f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16_t)0x7fc0;  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                             // round to nearest even
+    return (bf16_t)(u >> 16);
+}
+
+__device__ __forceinline__ int wrap(int i, int D) { return i < 0 ? i + D : (i >= D ? i - D : i); }
+
+// 8 bf16 of x kept where the matching mask value is > 0 (ReLU derivative of
+// the forward output), else 0.
+__device__ __forceinline__ u32x4 mask8(u32x4 x, u32x4 m) {
+    u32x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const unsigned lo = m[i] & 0xffffu, hi = m[i] >> 16;
+        const unsigned klo = (lo != 0 && !(lo & 0x8000u)) ? 0x0000ffffu : 0u;
+        const unsigned khi = (hi != 0 && !(hi & 0x8000u)) ? 0xffff0000u : 0u;
+        r[i] = x[i] & (klo | khi);
+    }
+    return r;
+}
+
+__device__ __forceinline__ bool pos_bf(bf16_t m) { return m != 0 && !(m & 0x8000u); }
+
+// ------------------------------------------------------------------ 3^3 conv (MFMA)
+
+struct ConvArgs {
+    int B, D, c0, o0;
+    const bf16_t* x0;
+    const bf16_t* x1;
+    const bf16_t* mask;
+    const bf16_t* w;  // [cout][27][cin]
+    const float* bias;
+    int bias_stride;  // 0 (shared) or cout (per sample)
+    int relu;
+    bf16_t* y0;
+    bf16_t* y1;
+};
+
+template <int CIN, int COUT, int TZ, int TY>
+__global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
+    constexpr int HZ = TZ + 2, HY = TY + 2, HX = 18, CP = CIN + 8;
+    constexpr int MT = TZ * TY, MW = MT / 4, NT = COUT / 16;
+    constexpr int K = 27 * CIN, KC = (K + 31) / 32;
+    static_assert(MT % 4 == 0, "four waves share the M-tiles");
+    static_assert(CIN % 8 == 0 && COUT % 16 == 0, "channel counts");
+    __shared__ __attribute__((aligned(16))) bf16_t tile[HZ * HY * HX * CP];
+    const int D = a.D;
+    const int nyt = D / TY;
+    const int x0 = blockIdx.x * 16;
+    const int y0 = (blockIdx.y % nyt) * TY, z0 = (blockIdx.y / nyt) * TZ;
+    const int b = blockIdx.z;
+    const int c1 = CIN - a.c0;
+    for (int i = threadIdx.x; i < HZ * HY * HX * (CIN / 8); i += 256) {
+        const int ch = (i % (CIN / 8)) * 8;
+        const int vox = i / (CIN / 8);
+        const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
+        const size_t g = (((size_t)b * D + wrap(z0 + hz - 1, D)) * D + wrap(y0 + hy - 1, D)) * D + wrap(x0 + hx - 1, D);
+        u32x4 v;
+        if (ch < a.c0) {
+            v = *(const u32x4*)(a.x0 + g * a.c0 + ch);
+            if (a.mask) v = mask8(v, *(const u32x4*)(a.mask + g * a.c0 + ch));
+        } else {
+            v = *(const u32x4*)(a.x1 + g * c1 + (ch - a.c0));
+        }
+        *(u32x4*)(tile + vox * CP + ch) = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, grp = lane >> 4;
+    floatx4 acc[MW][NT];
+#pragma unroll
+    for (int m = 0; m < MW; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[m][n] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    const u32x4 zero = {0u, 0u, 0u, 0u};
+    for (int kc = 0; kc < KC; ++kc) {
+        // K is padded to a multiple of 32 with zeros; addresses of the padding
+        // are clamped to tap 0 so that every access stays inside the tile and
+        // the weight array
+        const int kraw = kc * 32 + grp * 8;
+        const bool valid = kraw < K;
+        const int k0 = valid ? kraw : 0;
+        const int tap = k0 / CIN, ci = k0 % CIN;
+        bf16x8 bw[NT];
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+            bw[n] = __builtin_bit_cast(bf16x8, valid ? *(const u32x4*)(a.w + (size_t)(n * 16 + row) * K + k0) : zero);
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+#pragma unroll
+        for (int m = 0; m < MW; ++m) {
+            const int mt = wave * MW + m;
+            const int lz = mt / TY, ly = mt % TY;
+            const int off = (((lz + dz) * HY + (ly + dy)) * HX + (row + dx)) * CP + ci;
+            const bf16x8 av = __builtin_bit_cast(bf16x8, valid ? *(const u32x4*)(tile + off) : zero);
+#pragma unroll
+            for (int n = 0; n < NT; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw[n], acc[m][n], 0, 0, 0);
+        }
+    }
+    const int o1 = COUT - a.o0;
+#pragma unroll
+    for (int m = 0; m < MW; ++m) {
+        const int mt = wave * MW + m;
+        const int z = z0 + mt / TY, y = y0 + mt % TY;
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const int co = n * 16 + row;
+            const float bv = a.bias ? a.bias[b * a.bias_stride + co] : 0.0f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int x = x0 + grp * 4 + i;
+                float v = acc[m][n][i] + bv;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                const size_t vox = (((size_t)b * D + z) * D + y) * D + x;
+                if (co < a.o0)
+                    a.y0[vox * a.o0 + co] = f2bf(v);
+                else
+                    a.y1[vox * o1 + (co - a.o0)] = f2bf(v);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ 3^3 conv weight gradient
+
+struct WgradArgs {
+    int B, D, c0;
+    const bf16_t* x0;
+    const bf16_t* x1;
+    const bf16_t* g;
+    const bf16_t* mask;
+    float* partial;  // [gridDim.x][cout][27][cin]
+    int tiles_per_block;
+};
+
+// Voxel tile 2 x 2 x 16 (64 voxels); taps in groups of 3 (grid.y = 9).  Each
+// thread owns 4 output x 4 input channels for the group's 3 taps; thread sets
+// split the voxels and are added in LDS at the end.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_conv3_wgrad(WgradArgs a) {
+    constexpr int TZ = 2, TY = 2, HZ = 4, HY = 4, HX = 18, NV = TZ * TY * 16;
+    constexpr int PAIRS = (COUT / 4) * (CIN / 4);
+    constexpr int SETS = 256 / PAIRS;
+    static_assert(PAIRS <= 256 && 256 % PAIRS == 0, "pairs per block");
+    __shared__ __attribute__((aligned(16))) bf16_t xt[HZ * HY * HX * CIN];
+    __shared__ float gt[NV * COUT];
+    __shared__ float red[(SETS > 1 ? 256 : 1) * 16];
+    const int D = a.D;
+    const int tg = blockIdx.y;  // taps 3*tg .. 3*tg+2
+    const int t = threadIdx.x;
+    const int pair = t % PAIRS, set = t / PAIRS;
+    const int co0 = (pair % (COUT / 4)) * 4, ci0 = (pair / (COUT / 4)) * 4;
+    const int c1 = CIN - a.c0;
+    const int nx = D / 16, ny = D / TY, nz = D / TZ;
+    const int ntiles = a.B * nz * ny * nx;
+    float acc[3][4][4];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[q][i][j] = 0.0f;
+    for (int s = 0; s < a.tiles_per_block; ++s) {
+        const int tile = blockIdx.x * a.tiles_per_block + s;
+        if (tile >= ntiles) break;  // uniform
+        const int xt0 = (tile % nx) * 16;
+        const int yt0 = ((tile / nx) % ny) * TY;
+        const int zt0 = ((tile / (nx * ny)) % nz) * TZ;
+        const int b = tile / (nx * ny * nz);
+        __syncthreads();  // previous tile fully consumed
+        for (int i = t; i < HZ * HY * HX * (CIN / 8); i += 256) {
+            const int ch = (i % (CIN / 8)) * 8;
+            const int vox = i / (CIN / 8);
+            const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
+            const size_t g = (((size_t)b * D + wrap(zt0 + hz - 1, D)) * D + wrap(yt0 + hy - 1, D)) * D +
+                             wrap(xt0 + hx - 1, D);
+            const u32x4 v = ch < a.c0 ? *(const u32x4*)(a.x0 + g * a.c0 + ch)
+                                      : *(const u32x4*)(a.x1 + g * c1 + (ch - a.c0));
+            *(u32x4*)(xt + vox * CIN + ch) = v;
+        }
+        for (int i = t; i < NV * COUT; i += 256) {
+            const int co = i % COUT, v = i / COUT;
+            const int lx = v % 16, ly = (v / 16) % TY, lz = v / (16 * TY);
+            const size_t g = (((size_t)b * D + zt0 + lz) * D + yt0 + ly) * D + xt0 + lx;
+            float gv = bf2f(a.g[g * COUT + co]);
+            if (a.mask && !pos_bf(a.mask[g * COUT + co])) gv = 0.0f;
+            gt[i] = gv;
+        }
+        __syncthreads();
+        for (int v = set; v < NV; v += SETS) {
+            const int lx = v % 16, ly = (v / 16) % TY, lz = v / (16 * TY);
+            float gv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) gv[i] = gt[v * COUT + co0 + i];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int tap = tg * 3 + q;
+                const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+                const bf16_t* xp = xt + (((lz + dz) * HY + (ly + dy)) * HX + (lx + dx)) * CIN + ci0;
+                float xv[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xv[j] = bf2f(xp[j]);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[q][i][j] = fmaf(gv[i], xv[j], acc[q][i][j]);
+            }
+        }
+    }
+    // add the sets (fixed order) and write this block's partial
+    float* out = a.partial + (size_t)blockIdx.x * COUT * 27 * CIN;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int tap = tg * 3 + q;
+        if (SETS > 1) {
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) red[t * 16 + i * 4 + j] = acc[q][i][j];
+            __syncthreads();
+            if (set == 0) {
+                for (int s2 = 1; s2 < SETS; ++s2)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc[q][i][j] += red[(s2 * PAIRS + pair) * 16 + i * 4 + j];
+            }
+        }
+        if (set == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) out[((size_t)(co0 + i) * 27 + tap) * CIN + ci0 + j] = acc[q][i][j];
+        }
+    }
+}
+
+// Fixed-order sum over blocks of n-float partials: out[i] = sum_b partial[b][i].
+__global__ void k_sum_blocks(const float* __restrict__ partial, int nblk, int n, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * n + i];
+    out[i] = s;
+}
+
+// Per-sample channel sums of a (masked) gradient: partial[b][blk][c].
+__global__ void k_channel_partials(const bf16_t* __restrict__ g, const bf16_t* __restrict__ mask, int C,
+                                   size_t vox_per_sample, int blocks_per_sample, float* __restrict__ partial) {
+    __shared__ float red[256];
+    const int b = blockIdx.y;
+    const int c = threadIdx.x % C;  // C divides 256
+    const int lanes = 256 / C;
+    const int l = threadIdx.x / C;
+    float s = 0.0f;
+    const size_t per = (vox_per_sample + blocks_per_sample - 1) / blocks_per_sample;
+    const size_t v0 = (size_t)blockIdx.x * per;
+    const size_t v1 = v0 + per < vox_per_sample ? v0 + per : vox_per_sample;
+    for (size_t v = v0 + l; v < v1; v += lanes) {
+        const size_t idx = ((size_t)b * vox_per_sample + v) * C + c;
+        float gv = bf2f(g[idx]);
+        if (mask && !pos_bf(mask[idx])) gv = 0.0f;
+        s += gv;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        float t = 0.0f;
+        for (int q = 0; q < lanes; ++q) t += red[q * C + threadIdx.x];
+        partial[((size_t)b * blocks_per_sample + blockIdx.x) * C + threadIdx.x] = t;
+    }
+}
+
+// ------------------------------------------------------------------ pooling
+
+__global__ void k_pool_fwd(const bf16_t* __restrict__ x, bf16_t* __restrict__ y, int B, int D, int C) {
+    const int Dh = D / 2;
+    const size_t n = (size_t)B * Dh * Dh * Dh * C;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = i % C;
+        size_t v = i / C;
+        const int x0 = v % Dh, y0 = (v / Dh) % Dh, z0 = (v / ((size_t)Dh * Dh)) % Dh;
+        const int b = v / ((size_t)Dh * Dh * Dh);
+        float s = 0.0f;
+        for (int q = 0; q < 8; ++q) {
+            const size_t u = (((size_t)b * D + 2 * z0 + (q >> 2)) * D + 2 * y0 + ((q >> 1) & 1)) * D + 2 * x0 + (q & 1);
+            s += bf2f(x[u * C + c]);
+        }
+        y[i] = f2bf(s * 0.125f);
+    }
+}
+
+__global__ void k_pool_bwd(const bf16_t* __restrict__ dy, bf16_t* __restrict__ dx, int B, int D, int C, int add) {
+    const int Dh = D / 2;
+    const size_t n = (size_t)B * D * D * D * C;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = i % C;
+        size_t v = i / C;
+        const int x = v % D, y = (v / D) % D, z = (v / ((size_t)D * D)) % D;
+        const int b = v / ((size_t)D * D * D);
+        const size_t u = (((size_t)b * Dh + z / 2) * Dh + y / 2) * Dh + x / 2;
+        float g = bf2f(dy[u * C + c]) * 0.125f;
+        if (add) g += bf2f(dx[i]);
+        dx[i] = f2bf(g);
+    }
+}
+
+// ------------------------------------------------------------------ 2^3 stride-2 transposed conv
+
+// y[2v+s][co] = bias[co] + sum_ci w[co][s][ci] x[v][ci]; one thread per (input voxel, s).
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_up_fwd(const bf16_t* __restrict__ x, const float* __restrict__ w,
+                                                const float* __restrict__ bias, bf16_t* __restrict__ y, int B, int D) {
+    __shared__ float ws[COUT * 8 * CIN];
+    for (int i = threadIdx.x; i < COUT * 8 * CIN; i += 256) ws[i] = w[i];
+    __syncthreads();
+    const size_t nv = (size_t)B * D * D * D;
+    const size_t item = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (item >= nv * 8) return;
+    const int s = item % 8;
+    const size_t v = item / 8;
+    float xv[CIN];
+#pragma unroll
+    for (int c = 0; c < CIN; ++c) xv[c] = bf2f(x[v * CIN + c]);
+    const int xx = v % D, yy = (v / D) % D, zz = (v / ((size_t)D * D)) % D;
+    const int b = v / ((size_t)D * D * D);
+    const int D2 = 2 * D;
+    const size_t u = (((size_t)b * D2 + 2 * zz + (s >> 2)) * D2 + 2 * yy + ((s >> 1) & 1)) * D2 + 2 * xx + (s & 1);
+    for (int co = 0; co < COUT; ++co) {
+        float acc = bias[co];
+        const float* wp = ws + (co * 8 + s) * CIN;
+#pragma unroll
+        for (int c = 0; c < CIN; ++c) acc = fmaf(wp[c], xv[c], acc);
+        y[u * COUT + co] = f2bf(acc);
+    }
+}
+
+// dx[v][ci] = sum_{s,co} w[co][s][ci] dy[2v+s][co]; one thread per input voxel.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_up_bwd_data(const bf16_t* __restrict__ dy, const float* __restrict__ w,
+                                                     bf16_t* __restrict__ dx, int B, int D) {
+    __shared__ float ws[COUT * 8 * CIN];
+    for (int i = threadIdx.x; i < COUT * 8 * CIN; i += 256) ws[i] = w[i];
+    __syncthreads();
+    const size_t nv = (size_t)B * D * D * D;
+    const size_t v = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= nv) return;
+    const int xx = v % D, yy = (v / D) % D, zz = (v / ((size_t)D * D)) % D;
+    const int b = v / ((size_t)D * D * D);
+    const int D2 = 2 * D;
+    float acc[CIN];
+#pragma unroll
+    for (int c = 0; c < CIN; ++c) acc[c] = 0.0f;
+    for (int s = 0; s < 8; ++s) {
+        const size_t u = (((size_t)b * D2 + 2 * zz + (s >> 2)) * D2 + 2 * yy + ((s >> 1) & 1)) * D2 + 2 * xx + (s & 1);
+        for (int co = 0; co < COUT; ++co) {
+            const float g = bf2f(dy[u * COUT + co]);
+            const float* wp = ws + (co * 8 + s) * CIN;
+#pragma unroll
+            for (int c = 0; c < CIN; ++c) acc[c] = fmaf(wp[c], g, acc[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CIN; ++c) dx[v * CIN + c] = f2bf(acc[c]);
+}
+
+// dW[co][s][ci] = sum_v dy[2v+s][co] x[v][ci]: block partials over voxel
+// ranges; each thread owns 4 x 4 (co, ci) for all 8 s; sets split voxels.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_up_wgrad(const bf16_t* __restrict__ x, const bf16_t* __restrict__ dy,
+                                                  float* __restrict__ partial, int B, int D, int vox_per_block) {
+    constexpr int PAIRS = (COUT / 4) * (CIN / 4);
+    constexpr int SETS = 256 / PAIRS;
+    static_assert(PAIRS <= 256 && 256 % PAIRS == 0, "pairs per block");
+    __shared__ float red[(SETS > 1 ? 256 : 1) * 16];
+    const int t = threadIdx.x;
+    const int pair = t % PAIRS, set = t / PAIRS;
+    const int co0 = (pair % (COUT / 4)) * 4, ci0 = (pair / (COUT / 4)) * 4;
+    const size_t nv = (size_t)B * D * D * D;
+    const size_t v0 = (size_t)blockIdx.x * vox_per_block;
+    const size_t v1 = v0 + vox_per_block < nv ? v0 + vox_per_block : nv;
+    const int D2 = 2 * D;
+    float* out = partial + (size_t)blockIdx.x * COUT * 8 * CIN;
+    for (int s = 0; s < 8; ++s) {
+        float acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+        for (size_t v = v0 + set; v < v1; v += SETS) {
+            const int xx = v % D, yy = (v / D) % D, zz = (v / ((size_t)D * D)) % D;
+            const int b = v / ((size_t)D * D * D);
+            const size_t u =
+                (((size_t)b * D2 + 2 * zz + (s >> 2)) * D2 + 2 * yy + ((s >> 1) & 1)) * D2 + 2 * xx + (s & 1);
+            float gv[4], xv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) gv[i] = bf2f(dy[u * COUT + co0 + i]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xv[j] = bf2f(x[v * CIN + ci0 + j]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(gv[i], xv[j], acc[i][j]);
+        }
+        if (SETS > 1) {
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) red[t * 16 + i * 4 + j] = acc[i][j];
+            __syncthreads();
+            if (set == 0)
+                for (int s2 = 1; s2 < SETS; ++s2)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc[i][j] += red[(s2 * PAIRS + pair) * 16 + i * 4 + j];
+        }
+        if (set == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) out[((size_t)(co0 + i) * 8 + s) * CIN + ci0 + j] = acc[i][j];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ input layer (density + conditioning)
+
+// beff[b][co] = bias[co] + cond[b][0] sum_t w[co][t][1] + cond[b][1] sum_t w[co][t][2]
+__global__ void k_in_bias(const float* __restrict__ w, const float* __restrict__ bias, const float* __restrict__ cond,
+                          float* __restrict__ beff, int B, int COUT) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B * COUT) return;
+    const int b = i / COUT, co = i % COUT;
+    float s1 = 0.0f, s2 = 0.0f;
+    for (int t = 0; t < 27; ++t) {
+        s1 += w[(co * 27 + t) * 3 + 1];
+        s2 += w[(co * 27 + t) * 3 + 2];
+    }
+    beff[i] = bias[co] + cond[b * 2 + 0] * s1 + cond[b * 2 + 1] * s2;
+}
+
+// y[v][co] = relu(beff[b][co] + sum_t w[co][t][0] * scale * delta[v + off_t]); one thread per voxel.
+template <int COUT>
+__global__ __launch_bounds__(256) void k_in_fwd(const float* __restrict__ delta, const float* __restrict__ w,
+                                                const float* __restrict__ beff, bf16_t* __restrict__ y, int B, int D,
+                                                float scale) {
+    __shared__ float ws[COUT * 27];
+    for (int i = threadIdx.x; i < COUT * 27; i += 256) ws[i] = w[i * 3];
+    __syncthreads();
+    const size_t nv = (size_t)B * D * D * D;
+    const size_t v = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= nv) return;
+    const int xx = v % D, yy = (v / D) % D, zz = (v / ((size_t)D * D)) % D;
+    const int b = v / ((size_t)D * D * D);
+    float d[27];
+    for (int t = 0; t < 27; ++t) {
+        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+        d[t] = scale * delta[(((size_t)b * D + wrap(zz + dz, D)) * D + wrap(yy + dy, D)) * D + wrap(xx + dx, D)];
+    }
+    for (int co = 0; co < COUT; ++co) {
+        float acc = beff[b * COUT + co];
+#pragma unroll
+        for (int t = 0; t < 27; ++t) acc = fmaf(ws[co * 27 + t], d[t], acc);
+        y[v * COUT + co] = f2bf(fmaxf(acc, 0.0f));
+    }
+}
+
+// partial[blk][co][t] = sum_v g'[v][co] * scale * delta[v + off_t] with
+// g' = g masked by y > 0, over voxel tiles of 4 x 4 x 16 (periodic halo of
+// delta and the masked gradient in LDS); each thread owns (co, tap) pairs.
+template <int COUT>
+__global__ __launch_bounds__(256) void k_in_wgrad(const float* __restrict__ delta, const bf16_t* __restrict__ g,
+                                                  const bf16_t* __restrict__ y, float* __restrict__ partial, int B,
+                                                  int D, float scale, int tiles_per_block) {
+    constexpr int TZ = 4, TY = 4, HZ = 6, HY = 6, HX = 18, NV = TZ * TY * 16, NP = COUT * 27;
+    constexpr int PP = (NP + 255) / 256;
+    __shared__ float dt[HZ * HY * HX];
+    __shared__ float gt[NV * COUT];
+    const int t = threadIdx.x;
+    const int nx = D / 16, ny = D / TY, nz = D / TZ;
+    const int ntiles = B * nz * ny * nx;
+    float acc[PP];
+#pragma unroll
+    for (int q = 0; q < PP; ++q) acc[q] = 0.0f;
+    for (int s = 0; s < tiles_per_block; ++s) {
+        const int tile = blockIdx.x * tiles_per_block + s;
+        if (tile >= ntiles) break;  // uniform
+        const int xt0 = (tile % nx) * 16;
+        const int yt0 = ((tile / nx) % ny) * TY;
+        const int zt0 = ((tile / (nx * ny)) % nz) * TZ;
+        const int b = tile / (nx * ny * nz);
+        __syncthreads();
+        for (int i = t; i < HZ * HY * HX; i += 256) {
+            const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
+            dt[i] = scale * delta[(((size_t)b * D + wrap(zt0 + hz - 1, D)) * D + wrap(yt0 + hy - 1, D)) * D +
+                                  wrap(xt0 + hx - 1, D)];
+        }
+        for (int i = t; i < NV * COUT; i += 256) {
+            const int co = i % COUT, v = i / COUT;
+            const int lx = v % 16, ly = (v / 16) % TY, lz = v / (16 * TY);
+            const size_t idx = ((((size_t)b * D + zt0 + lz) * D + yt0 + ly) * D + xt0 + lx) * COUT + co;
+            gt[i] = pos_bf(y[idx]) ? bf2f(g[idx]) : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PP; ++q) {
+            const int p = t + q * 256;
+            if (p < NP) {
+                const int co = p / 27, tap = p % 27;
+                const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+                float a = acc[q];
+                for (int v = 0; v < NV; ++v) {
+                    const int lx = v % 16, ly = (v / 16) % TY, lz = v / (16 * TY);
+                    a = fmaf(gt[v * COUT + co], dt[((lz + dz) * HY + (ly + dy)) * HX + lx + dx], a);
+                }
+                acc[q] = a;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < PP; ++q) {
+        const int p = t + q * 256;
+        if (p < NP) partial[(size_t)blockIdx.x * NP + p] = acc[q];
+    }
+}
+
+// Full input-layer weight gradient [co][t][3] and bias gradient from the
+// density part (dwd[co][t]) and the per-sample gradient sums gsum[b][co]:
+// the conditioning channels are constant, so their gradient is the same for
+// every tap: sum_b cond[b][k] gsum[b][co].
+__global__ void k_in_finish(const float* __restrict__ dwd, const float* __restrict__ gsum,
+                            const float* __restrict__ cond, float* __restrict__ dw, float* __restrict__ db, int B,
+                            int COUT) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= COUT * 27) return;
+    const int co = i / 27;
+    float s1 = 0.0f, s2 = 0.0f, s0 = 0.0f;
+    for (int b = 0; b < B; ++b) {
+        s0 += gsum[b * COUT + co];
+        s1 += cond[b * 2 + 0] * gsum[b * COUT + co];
+        s2 += cond[b * 2 + 1] * gsum[b * COUT + co];
+    }
+    dw[i * 3 + 0] = dwd[i];
+    dw[i * 3 + 1] = s1;
+    dw[i * 3 + 2] = s2;
+    if (i % 27 == 0) db[co] = s0;
+}
+
+// ------------------------------------------------------------------ head: 1^3 conv, sigmoid, BCE
+
+// logit = b + sum_c w[c] x[v][c]; p = sigmoid(logit).  With a target:
+// loss += -(t log p + (1-t) log(1-p)) (FP64 partial per block),
+// dlogit = (p - t) * inv_n, dx[v][c] = dlogit * w[c],
+// partial dw[c] += dlogit x[v][c], db += dlogit.
+template <int C>
+__global__ __launch_bounds__(256) void k_head(const bf16_t* __restrict__ x, const float* __restrict__ w,
+                                              const float* __restrict__ bias, const float* __restrict__ target,
+                                              float* __restrict__ prob, bf16_t* __restrict__ dx,
+                                              float* __restrict__ partial, double* __restrict__ lpart, size_t nv,
+                                              float inv_n) {
+    __shared__ float red[256 * (C + 1)];
+    __shared__ double lred[256];
+    const int t = threadIdx.x;
+    float wv[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) wv[c] = w[c];
+    const float b0 = bias[0];
+    float gw[C + 1];
+#pragma unroll
+    for (int c = 0; c <= C; ++c) gw[c] = 0.0f;
+    double loss = 0.0;
+    for (size_t v = (size_t)blockIdx.x * 256 + t; v < nv; v += (size_t)gridDim.x * 256) {
+        float xv[C];
+        const u32x4* xp = (const u32x4*)(x + v * C);
+#pragma unroll
+        for (int q = 0; q < C / 8; ++q) {
+            const u32x4 r = xp[q];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                xv[q * 8 + 2 * k] = __uint_as_float(r[k] << 16);
+                xv[q * 8 + 2 * k + 1] = __uint_as_float(r[k] & 0xffff0000u);
+            }
+        }
+        float lg = b0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) lg = fmaf(wv[c], xv[c], lg);
+        const float p = 1.0f / (1.0f + expf(-lg));
+        if (prob) prob[v] = p;
+        if (target) {
+            const float tv = target[v];
+            // log p = -softplus(-lg), log(1-p) = -softplus(lg), stable for large |lg|
+            const double lp = -(double)(fmaxf(-lg, 0.0f) + log1pf(expf(-fabsf(lg))));
+            const double lq = -(double)(fmaxf(lg, 0.0f) + log1pf(expf(-fabsf(lg))));
+            loss -= (double)tv * lp + (1.0 - (double)tv) * lq;
+            const float dl = (p - tv) * inv_n;
+            if (dx) {
+#pragma unroll
+                for (int c = 0; c < C; ++c) dx[v * C + c] = f2bf(dl * wv[c]);
+            }
+#pragma unroll
+            for (int c = 0; c < C; ++c) gw[c] = fmaf(dl, xv[c], gw[c]);
+            gw[C] += dl;
+        }
+    }
+    if (!target) return;
+#pragma unroll
+    for (int c = 0; c <= C; ++c) red[c * 256 + t] = gw[c];
+    lred[t] = loss;
+    __syncthreads();
+    if (t <= C) {
+        float s = 0.0f;
+        for (int q = 0; q < 256; ++q) s += red[t * 256 + q];
+        partial[(size_t)blockIdx.x * (C + 1) + t] = s;
+    }
+    if (t == 0) {
+        double s = 0.0;
+        for (int q = 0; q < 256; ++q) s += lred[q];
+        lpart[blockIdx.x] = s;
+    }
+}
+
+__global__ void k_sum_f64(const double* __restrict__ p, int n, double scale, double* __restrict__ out) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) *out = s * scale;
+}
+
+// ------------------------------------------------------------------ packing and AdamW
+
+// wb[co][t][ci] = bf16(w[co][t][ci]);  wt[ci][t'][co] = bf16(w[co][26-t'][ci])
+__global__ void k_pack(const float* __restrict__ w, bf16_t* __restrict__ wb, bf16_t* __restrict__ wt, int cin,
+                       int cout) {
+    const int n = cout * 27 * cin;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ci = i % cin, t = (i / cin) % 27, co = i / (27 * cin);
+    const bf16_t h = f2bf(w[i]);
+    if (wb) wb[i] = h;
+    if (wt) wt[((size_t)ci * 27 + (26 - t)) * cout + co] = h;
+}
+
+__global__ void k_adamw(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                        float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps, float wd, float c1,
+                        float c2) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float mh = mi / c1, vh = vi / c2;
+        p[i] = p[i] - lr * (mh / (sqrtf(vh) + eps) + wd * p[i]);
+    }
+}
+
+// ------------------------------------------------------------------ dispatch
+
+using ConvFn = void (*)(const ConvArgs&, hipStream_t);
+template <int CIN, int COUT, int TZ, int TY>
+void launch_conv(const ConvArgs& a, hipStream_t st) {
+    const dim3 grid(a.D / 16, (a.D / TZ) * (a.D / TY), a.B);
+    k_conv3_mfma<CIN, COUT, TZ, TY><<<grid, 256, 0, st>>>(a);
+}
+
+struct ConvShape {
+    int cin, cout;
+    ConvFn big, small;  // D >= 64 / D < 64
+    int tz_big, ty_big, tz_small, ty_small;
+};
+
+const ConvShape kConvShapes[] = {
+    {16, 16, launch_conv<16, 16, 4, 4>, launch_conv<16, 16, 2, 2>, 4, 4, 2, 2},
+    {16, 32, launch_conv<16, 32, 4, 4>, launch_conv<16, 32, 2, 2>, 4, 4, 2, 2},
+    {32, 16, launch_conv<32, 16, 4, 4>, launch_conv<32, 16, 2, 2>, 4, 4, 2, 2},
+    {32, 32, launch_conv<32, 32, 4, 4>, launch_conv<32, 32, 2, 2>, 4, 4, 2, 2},
+    {32, 64, launch_conv<32, 64, 4, 2>, launch_conv<32, 64, 2, 2>, 4, 2, 2, 2},
+    {64, 32, launch_conv<64, 32, 4, 2>, launch_conv<64, 32, 2, 2>, 4, 2, 2, 2},
+    {64, 64, launch_conv<64, 64, 4, 2>, launch_conv<64, 64, 2, 2>, 4, 2, 2, 2},
+};
+
+const ConvShape* conv_shape(int cin, int cout) {
+    for (const auto& s : kConvShapes)
+        if (s.cin == cin && s.cout == cout) return &s;
+    return nullptr;
+}
+
+using WgradFn = void (*)(const WgradArgs&, int nblk, hipStream_t);
+template <int CIN, int COUT>
+void launch_wgrad(const WgradArgs& a, int nblk, hipStream_t st) {
+    k_conv3_wgrad<CIN, COUT><<<dim3(nblk, 9), 256, 0, st>>>(a);
+}
+struct WgradShape {
+    int cin, cout;
+    WgradFn fn;
+};
+const WgradShape kWgradShapes[] = {
+    {16, 16, launch_wgrad<16, 16>}, {16, 32, launch_wgrad<16, 32>}, {32, 16, launch_wgrad<32, 16>},
+    {32, 32, launch_wgrad<32, 32>}, {32, 64, launch_wgrad<32, 64>}, {64, 32, launch_wgrad<64, 32>},
+    {64, 64, launch_wgrad<64, 64>},
+};
+const WgradShape* wgrad_shape(int cin, int cout) {
+    for (const auto& s : kWgradShapes)
+        if (s.cin == cin && s.cout == cout) return &s;
+    return nullptr;
+}
+
+constexpr int kWgradBlocks = 256;   // voxel-slab blocks of the weight gradient
+constexpr int kChanBlocks = 64;     // per-sample blocks of the channel sums
+constexpr int kHeadBlocks = 1024;
+constexpr int kUpBlocksMax = 256;
+
+bool valid_unet_dim(int D) { return D >= 16 && D <= 1024 && D % 16 == 0; }
+
+}  // namespace
+}  // namespace cm21
+
+using namespace cm21;
+
+extern "C" int cm21_unet_conv3_fwd(const cm21_unet_conv* c, void* stream) {
+    CM21_REQUIRE(c, "cm21_unet_conv3_fwd: NULL descriptor");
+    const ConvShape* s = conv_shape(c->cin, c->cout);
+    CM21_REQUIRE(s, "cm21_unet_conv3_fwd: no kernel for cin=%d cout=%d", c->cin, c->cout);
+    CM21_REQUIRE(c->B >= 1 && valid_unet_dim(c->D), "cm21_unet_conv3_fwd: B=%d D=%d", c->B, c->D);
+    CM21_REQUIRE(c->c0 > 0 && c->c0 <= c->cin && c->c0 % 8 == 0, "cm21_unet_conv3_fwd: c0=%d", c->c0);
+    CM21_REQUIRE(c->o0 > 0 && c->o0 <= c->cout && c->o0 % 16 == 0, "cm21_unet_conv3_fwd: o0=%d", c->o0);
+    CM21_REQUIRE(c->x0 && c->w && c->y0, "cm21_unet_conv3_fwd: NULL x0, w or y0");
+    CM21_REQUIRE(c->c0 == c->cin || c->x1, "cm21_unet_conv3_fwd: x1 is NULL with c0 < cin");
+    CM21_REQUIRE(c->o0 == c->cout || c->y1, "cm21_unet_conv3_fwd: y1 is NULL with o0 < cout");
+    CM21_REQUIRE(!c->mask || c->c0 == c->cin, "cm21_unet_conv3_fwd: mask needs a single input");
+    ConvArgs a{};
+    a.B = c->B;
+    a.D = c->D;
+    a.c0 = c->c0;
+    a.o0 = c->o0;
+    a.x0 = (const bf16_t*)c->x0;
+    a.x1 = (const bf16_t*)c->x1;
+    a.mask = (const bf16_t*)c->mask;
+    a.w = (const bf16_t*)c->w;
+    a.bias = c->bias;
+    a.bias_stride = c->bias_per_sample ? c->cout : 0;
+    a.relu = c->relu;
+    a.y0 = (bf16_t*)c->y0;
+    a.y1 = (bf16_t*)c->y1;
+    hipStream_t st = (hipStream_t)stream;
+    (c->D >= 64 ? s->big : s->small)(a, st);
+    CM21_LAUNCH_CHECK("k_conv3_mfma");
+    return CM21_OK;
+}
+
+extern "C" size_t cm21_unet_wgrad_workspace_bytes(int B, int D, int cin, int cout) {
+    if (B < 1 || !valid_unet_dim(D) || cin < 1 || cout < 1) return 0;
+    const size_t w = (size_t)kWgradBlocks * cout * 27 * cin * sizeof(float);
+    const size_t c = (size_t)B * kChanBlocks * cout * sizeof(float);
+    return align_up(w, 256) + align_up(c, 256);
+}
+
+extern "C" int cm21_unet_conv3_wgrad(const cm21_unet_wgrad* w, void* stream) {
+    CM21_REQUIRE(w, "cm21_unet_conv3_wgrad: NULL descriptor");
+    const WgradShape* s = wgrad_shape(w->cin, w->cout);
+    CM21_REQUIRE(s, "cm21_unet_conv3_wgrad: no kernel for cin=%d cout=%d", w->cin, w->cout);
+    CM21_REQUIRE(w->B >= 1 && valid_unet_dim(w->D), "cm21_unet_conv3_wgrad: B=%d D=%d", w->B, w->D);
+    CM21_REQUIRE(w->c0 > 0 && w->c0 <= w->cin && w->c0 % 8 == 0, "cm21_unet_conv3_wgrad: c0=%d", w->c0);
+    CM21_REQUIRE(w->x0 && w->g && w->dw && w->work, "cm21_unet_conv3_wgrad: NULL argument");
+    CM21_REQUIRE(w->c0 == w->cin || w->x1, "cm21_unet_conv3_wgrad: x1 is NULL with c0 < cin");
+    const size_t need = cm21_unet_wgrad_workspace_bytes(w->B, w->D, w->cin, w->cout);
+    if (w->work_bytes < need)
+        return fail(CM21_ERR_NOMEM, "cm21_unet_conv3_wgrad: workspace %zu < %zu", w->work_bytes, need);
+    hipStream_t st = (hipStream_t)stream;
+    float* partial = (float*)w->work;
+    float* cpart = (float*)((char*)w->work + align_up((size_t)kWgradBlocks * w->cout * 27 * w->cin * sizeof(float), 256));
+    const int ntiles = w->B * (w->D / 2) * (w->D / 2) * (w->D / 16);
+    const int tpb = (ntiles + kWgradBlocks - 1) / kWgradBlocks;
+    const int nblk = (ntiles + tpb - 1) / tpb;
+    WgradArgs a{};
+    a.B = w->B;
+    a.D = w->D;
+    a.c0 = w->c0;
+    a.x0 = (const bf16_t*)w->x0;
+    a.x1 = (const bf16_t*)w->x1;
+    a.g = (const bf16_t*)w->g;
+    a.mask = (const bf16_t*)w->mask;
+    a.partial = partial;
+    a.tiles_per_block = tpb;
+    s->fn(a, nblk, st);
+    CM21_LAUNCH_CHECK("k_conv3_wgrad");
+    const int n = w->cout * 27 * w->cin;
+    k_sum_blocks<<<(n + 255) / 256, 256, 0, st>>>(partial, nblk, n, w->dw);
+    CM21_LAUNCH_CHECK("k_sum_blocks");
+    if (w->db) {
+        const size_t vps = (size_t)w->D * w->D * w->D;
+        k_channel_partials<<<dim3(kChanBlocks, w->B), 256, 0, st>>>((const bf16_t*)w->g, (const bf16_t*)w->mask,
+                                                                     w->cout, vps, kChanBlocks, cpart);
+        CM21_LAUNCH_CHECK("k_channel_partials");
+        k_sum_blocks<<<1, 256, 0, st>>>(cpart, w->B * kChanBlocks, w->cout, w->db);
+        CM21_LAUNCH_CHECK("k_sum_blocks");
+    }
+    return CM21_OK;
+}
+
+extern "C" int cm21_unet_pool_fwd(const void* x, void* y, int B, int D, int C, void* stream) {
+    CM21_REQUIRE(x && y && B >= 1 && D >= 2 && D % 2 == 0 && C >= 1, "cm21_unet_pool_fwd: bad arguments");
+    const size_t n = (size_t)B * (D / 2) * (D / 2) * (D / 2) * C;
+    k_pool_fwd<<<grid_for(n), kThreads, 0, (hipStream_t)stream>>>((const bf16_t*)x, (bf16_t*)y, B, D, C);
+    CM21_LAUNCH_CHECK("k_pool_fwd");
+    return CM21_OK;
+}
+
+extern "C" int cm21_unet_pool_bwd(const void* dy, void* dx, int B, int D, int C, int accumulate, void* stream) {
+    CM21_REQUIRE(dy && dx && B >= 1 && D >= 2 && D % 2 == 0 && C >= 1, "cm21_unet_pool_bwd: bad arguments");
+    const size_t n = (size_t)B * D * D * D * C;
+    k_pool_bwd<<<grid_for(n), kThreads, 0, (hipStream_t)stream>>>((const bf16_t*)dy, (bf16_t*)dx, B, D, C,
+                                                                    accumulate ? 1 : 0);
+    CM21_LAUNCH_CHECK("k_pool_bwd");
+    return CM21_OK;
+}
+
+namespace {
+template <int CIN, int COUT>
+int up_fwd_t(const void* x, const float* w, const float* b, void* y, int B, int D, hipStream_t st) {
+    const size_t items = (size_t)B * D * D * D * 8;
+    k_up_fwd<CIN, COUT><<<(unsigned)((items + 255) / 256), 256, 0, st>>>((const bf16_t*)x, w, b, (bf16_t*)y, B, D);
+    CM21_LAUNCH_CHECK("k_up_fwd");
+    return CM21_OK;
+}
+template <int CIN, int COUT>
+int up_bwd_t(const void* x, const void* dy, const float* w, void* dx, float* dw, float* db, int B, int D, float* part,
+             float* cpart, hipStream_t st) {
+    const size_t nv = (size_t)B * D * D * D;
+    if (dx) {
+        k_up_bwd_data<CIN, COUT><<<(unsigned)((nv + 255) / 256), 256, 0, st>>>((const bf16_t*)dy, w, (bf16_t*)dx, B, D);
+        CM21_LAUNCH_CHECK("k_up_bwd_data");
+    }
+    const int vpb = (int)((nv + kUpBlocksMax - 1) / kUpBlocksMax);
+    const int nblk = (int)((nv + vpb - 1) / vpb);
+    k_up_wgrad<CIN, COUT><<<nblk, 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)dy, part, B, D, vpb);
+    CM21_LAUNCH_CHECK("k_up_wgrad");
+    const int n = COUT * 8 * CIN;
+    k_sum_blocks<<<(n + 255) / 256, 256, 0, st>>>(part, nblk, n, dw);
+    CM21_LAUNCH_CHECK("k_sum_blocks");
+    const size_t vps = (size_t)8 * D * D * D;
+    k_channel_partials<<<dim3(kChanBlocks, B), 256, 0, st>>>((const bf16_t*)dy, nullptr, COUT, vps, kChanBlocks, cpart);
+    CM21_LAUNCH_CHECK("k_channel_partials");
+    k_sum_blocks<<<1, 256, 0, st>>>(cpart, B * kChanBlocks, COUT, db);
+    CM21_LAUNCH_CHECK("k_sum_blocks");
+    return CM21_OK;
+}
+}  // namespace
+
+extern "C" int cm21_unet_up_fwd(const void* x, const float* w, const float* bias, void* y, int B, int D, int cin,
+                                int cout, void* stream) {
+    CM21_REQUIRE(x && w && bias && y && B >= 1 && D >= 2, "cm21_unet_up_fwd: bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    if (cin == 64 && cout == 32) return up_fwd_t<64, 32>(x, w, bias, y, B, D, st);
+    if (cin == 32 && cout == 16) return up_fwd_t<32, 16>(x, w, bias, y, B, D, st);
+    return fail(CM21_ERR_INVALID, "cm21_unet_up_fwd: no kernel for cin=%d cout=%d", cin, cout);
+}
+
+extern "C" size_t cm21_unet_up_workspace_bytes(int B, int D, int cin, int cout) {
+    if (B < 1 || D < 2 || cin < 1 || cout < 1) return 0;
+    return align_up((size_t)kUpBlocksMax * cout * 8 * cin * sizeof(float), 256) +
+           align_up((size_t)B * kChanBlocks * cout * sizeof(float), 256);
+}
+
+extern "C" int cm21_unet_up_bwd(const void* x, const void* dy, const float* w, void* dx, float* dw, float* db, int B,
+                                int D, int cin, int cout, void* work, size_t work_bytes, void* stream) {
+    CM21_REQUIRE(x && dy && w && dw && db && work && B >= 1 && D >= 2, "cm21_unet_up_bwd: bad arguments");
+    const size_t need = cm21_unet_up_workspace_bytes(B, D, cin, cout);
+    if (work_bytes < need) return fail(CM21_ERR_NOMEM, "cm21_unet_up_bwd: workspace %zu < %zu", work_bytes, need);
+    float* part = (float*)work;
+    float* cpart = (float*)((char*)work + align_up((size_t)kUpBlocksMax * cout * 8 * cin * sizeof(float), 256));
+    hipStream_t st = (hipStream_t)stream;
+    if (cin == 64 && cout == 32) return up_bwd_t<64, 32>(x, dy, w, dx, dw, db, B, D, part, cpart, st);
+    if (cin == 32 && cout == 16) return up_bwd_t<32, 16>(x, dy, w, dx, dw, db, B, D, part, cpart, st);
+    return fail(CM21_ERR_INVALID, "cm21_unet_up_bwd: no kernel for cin=%d cout=%d", cin, cout);
+}
+
+extern "C" int cm21_unet_in_fwd(const float* delta, const float* cond, const float* w, const float* bias, float* beff,
+                                void* y, int B, int D, float delta_scale, void* stream) {
+    CM21_REQUIRE(delta && cond && w && bias && beff && y && B >= 1 && valid_unet_dim(D),
+                 "cm21_unet_in_fwd: bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    k_in_bias<<<(B * 16 + 63) / 64, 64, 0, st>>>(w, bias, cond, beff, B, 16);
+    CM21_LAUNCH_CHECK("k_in_bias");
+    const size_t nv = (size_t)B * D * D * D;
+    k_in_fwd<16><<<(unsigned)((nv + 255) / 256), 256, 0, st>>>(delta, w, beff, (bf16_t*)y, B, D, delta_scale);
+    CM21_LAUNCH_CHECK("k_in_fwd");
+    return CM21_OK;
+}
+
+extern "C" size_t cm21_unet_in_workspace_bytes(int B, int D) {
+    if (B < 1 || !valid_unet_dim(D)) return 0;
+    return align_up((size_t)kWgradBlocks * 16 * 27 * sizeof(float), 256) +
+           align_up((size_t)B * kChanBlocks * 16 * sizeof(float), 256) + align_up((size_t)B * 16 * sizeof(float), 256) +
+           align_up((size_t)16 * 27 * sizeof(float), 256);
+}
+
+extern "C" int cm21_unet_in_wgrad(const float* delta, const float* cond, const void* g, const void* y, float* dw,
+                                  float* db, int B, int D, float delta_scale, void* work, size_t work_bytes,
+                                  void* stream) {
+    CM21_REQUIRE(delta && cond && g && y && dw && db && work && B >= 1 && valid_unet_dim(D),
+                 "cm21_unet_in_wgrad: bad arguments");
+    const size_t need = cm21_unet_in_workspace_bytes(B, D);
+    if (work_bytes < need) return fail(CM21_ERR_NOMEM, "cm21_unet_in_wgrad: workspace %zu < %zu", work_bytes, need);
+    hipStream_t st = (hipStream_t)stream;
+    char* p = (char*)work;
+    float* part = (float*)p;
+    p += align_up((size_t)kWgradBlocks * 16 * 27 * sizeof(float), 256);
+    float* cpart = (float*)p;
+    p += align_up((size_t)B * kChanBlocks * 16 * sizeof(float), 256);
+    float* gsum = (float*)p;
+    p += align_up((size_t)B * 16 * sizeof(float), 256);
+    float* dwd = (float*)p;
+    const int ntiles = B * (D / 4) * (D / 4) * (D / 16);
+    const int tpb = (ntiles + kWgradBlocks - 1) / kWgradBlocks;
+    const int nblk = (ntiles + tpb - 1) / tpb;
+    k_in_wgrad<16><<<nblk, 256, 0, st>>>(delta, (const bf16_t*)g, (const bf16_t*)y, part, B, D, delta_scale, tpb);
+    CM21_LAUNCH_CHECK("k_in_wgrad");
+    k_sum_blocks<<<(16 * 27 + 255) / 256, 256, 0, st>>>(part, nblk, 16 * 27, dwd);
+    CM21_LAUNCH_CHECK("k_sum_blocks");
+    const size_t vps = (size_t)D * D * D;
+    k_channel_partials<<<dim3(kChanBlocks, B), 256, 0, st>>>((const bf16_t*)g, (const bf16_t*)y, 16, vps, kChanBlocks,
+                                                              cpart);
+    CM21_LAUNCH_CHECK("k_channel_partials");
+    for (int b = 0; b < B; ++b) {
+        k_sum_blocks<<<1, 256, 0, st>>>(cpart + (size_t)b * kChanBlocks * 16, kChanBlocks, 16, gsum + b * 16);
+        CM21_LAUNCH_CHECK("k_sum_blocks");
+    }
+    k_in_finish<<<(16 * 27 + 255) / 256, 256, 0, st>>>(dwd, gsum, cond, dw, db, B, 16);
+    CM21_LAUNCH_CHECK("k_in_finish");
+    return CM21_OK;
+}
+
+extern "C" size_t cm21_unet_head_workspace_bytes(void) {
+    return align_up((size_t)kHeadBlocks * 17 * sizeof(float), 256) + align_up((size_t)kHeadBlocks * sizeof(double), 256);
+}
+
+extern "C" int cm21_unet_head(const void* x, const float* w, const float* bias, const float* target, float* prob,
+                              void* dx, float* dwb, double* loss, int B, int D, void* work, size_t work_bytes,
+                              void* stream) {
+    CM21_REQUIRE(x && w && bias && B >= 1 && valid_unet_dim(D), "cm21_unet_head: bad arguments");
+    CM21_REQUIRE(!target || (dwb && loss && work), "cm21_unet_head: a target needs dwb, loss and work");
+    if (target && work_bytes < cm21_unet_head_workspace_bytes())
+        return fail(CM21_ERR_NOMEM, "cm21_unet_head: workspace too small");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t nv = (size_t)B * D * D * D;
+    float* part = (float*)work;
+    double* lpart = work ? (double*)((char*)work + align_up((size_t)kHeadBlocks * 17 * sizeof(float), 256)) : nullptr;
+    k_head<16><<<kHeadBlocks, 256, 0, st>>>((const bf16_t*)x, w, bias, target, prob, (bf16_t*)dx, part, lpart, nv,
+                                            (float)(1.0 / (double)nv));
+    CM21_LAUNCH_CHECK("k_head");
+    if (target) {
+        k_sum_blocks<<<1, 256, 0, st>>>(part, kHeadBlocks, 17, dwb);
+        CM21_LAUNCH_CHECK("k_sum_blocks");
+        k_sum_f64<<<1, 256, 0, st>>>(lpart, kHeadBlocks, 1.0 / (double)nv, loss);
+        CM21_LAUNCH_CHECK("k_sum_f64");
+    }
+    return CM21_OK;
+}
+
+extern "C" int cm21_unet_pack(const float* w, void* wb, void* wt, int cin, int cout, void* stream) {
+    CM21_REQUIRE(w && (wb || wt) && cin >= 1 && cout >= 1, "cm21_unet_pack: bad arguments");
+    const int n = cout * 27 * cin;
+    k_pack<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(w, (bf16_t*)wb, (bf16_t*)wt, cin, cout);
+    CM21_LAUNCH_CHECK("k_pack");
+    return CM21_OK;
+}
+
+extern "C" int cm21_adamw(float* p, const float* g, float* m, float* v, size_t n, float lr, float beta1, float beta2,
+                          float eps, float weight_decay, int step, void* stream) {
+    CM21_REQUIRE(p && g && m && v && n > 0 && step >= 1, "cm21_adamw: bad arguments");
+    const float c1 = (float)(1.0 - std::pow((double)beta1, (double)step));
+    const float c2 = (float)(1.0 - std::pow((double)beta2, (double)step));
+    k_adamw<<<grid_for(n), kThreads, 0, (hipStream_t)stream>>>(p, g, m, v, n, lr, beta1, beta2, eps, weight_decay, c1,
+                                                               c2);
+    CM21_LAUNCH_CHECK("k_adamw");
+    return CM21_OK;
+}

@@ -268,6 +268,86 @@ const char* cm21_batch_tile_name(int D, int stage, int index);
 int cm21_batch_autotune(cm21_batch_group* groups, int ngroups, const cm21_bubble_params* params, void* work,
                         size_t work_bytes, int reps, double* ms_out, void* stream);
 
+/* ---- 3D U-Net surrogate layers.  No reference counterpart: a learned
+ * stand-in for the excursion-set loop (Programs/find_HII_bubbles.py:17-193),
+ * trained on its outputs.  Activations are bf16 (uint16 bit patterns),
+ * channels-last (B, D, D, D, C), periodic boundaries; master weights fp32,
+ * accumulation fp32.  3^3 conv weights are [cout][27][cin] with tap
+ * t = 9 (dz+1) + 3 (dy+1) + (dx+1). */
+typedef struct cm21_unet_conv {
+    int B, D;            /* batch, spatial side (input = output) */
+    int cin, cout;       /* channels; cin = c0 + (channels of x1) */
+    int c0;              /* input channels read from x0 (the rest from x1) */
+    int o0;              /* output channels written to y0 (the rest to y1) */
+    const void* x0;      /* bf16 (B,D,D,D,c0) */
+    const void* x1;      /* bf16 (B,D,D,D,cin-c0) or NULL */
+    const void* mask;    /* bf16 like x0 or NULL: x0 * (mask > 0) (single input only) */
+    const void* w;       /* bf16 [cout][27][cin] */
+    const float* bias;   /* fp32 [cout] ([B][cout] if bias_per_sample) or NULL */
+    int bias_per_sample;
+    int relu;            /* 1: y = max(0, conv + bias) */
+    void* y0;            /* bf16 (B,D,D,D,o0) */
+    void* y1;            /* bf16 (B,D,D,D,cout-o0) or NULL */
+} cm21_unet_conv;
+/* y = conv(x) on the matrix cores; (cin, cout) in {(16,16), (16,32), (32,16),
+ * (32,32), (32,64), (64,32), (64,64)}, D a multiple of 16.  The input
+ * gradient of a conv is this call on the masked output gradient with the
+ * weights from cm21_unet_pack's transposed copy. */
+int cm21_unet_conv3_fwd(const cm21_unet_conv* c, void* stream);
+
+typedef struct cm21_unet_wgrad {
+    int B, D, cin, cout, c0;
+    const void* x0;      /* forward input (bf16), c0 channels */
+    const void* x1;      /* forward input, cin - c0 channels, or NULL */
+    const void* g;       /* output gradient (bf16, cout channels) */
+    const void* mask;    /* forward output (bf16) or NULL: g * (mask > 0) */
+    float* dw;           /* fp32 [cout][27][cin], overwritten */
+    float* db;           /* fp32 [cout] or NULL, overwritten */
+    void* work;
+    size_t work_bytes;
+} cm21_unet_wgrad;
+size_t cm21_unet_wgrad_workspace_bytes(int B, int D, int cin, int cout);
+int cm21_unet_conv3_wgrad(const cm21_unet_wgrad* w, void* stream);
+
+/* 2x2x2 mean pooling: x (B,D,D,D,C) -> y (B,D/2,D/2,D/2,C); backward
+ * dx (+)= dy/8 broadcast (accumulate = 1 adds to dx). */
+int cm21_unet_pool_fwd(const void* x, void* y, int B, int D, int C, void* stream);
+int cm21_unet_pool_bwd(const void* dy, void* dx, int B, int D, int C, int accumulate, void* stream);
+
+/* 2x2x2 stride-2 transposed conv, x (B,D,D,D,cin) -> y (B,2D,2D,2D,cout),
+ * w fp32 [cout][8][cin], sub-voxel s = 4a + 2b + c; (cin, cout) in
+ * {(64,32), (32,16)}.  Backward: dx (or NULL), dw and db overwritten. */
+int cm21_unet_up_fwd(const void* x, const float* w, const float* bias, void* y, int B, int D, int cin, int cout,
+                     void* stream);
+size_t cm21_unet_up_workspace_bytes(int B, int D, int cin, int cout);
+int cm21_unet_up_bwd(const void* x, const void* dy, const float* w, void* dx, float* dw, float* db, int B, int D,
+                     int cin, int cout, void* work, size_t work_bytes, void* stream);
+
+/* Input layer: 16 channels from fp32 density delta (B,D,D,D) scaled by
+ * delta_scale and two constant conditioning channels cond[B][2]
+ * (w fp32 [16][27][3]); beff[B][16] receives the per-sample bias. */
+int cm21_unet_in_fwd(const float* delta, const float* cond, const float* w, const float* bias, float* beff,
+                     void* y, int B, int D, float delta_scale, void* stream);
+size_t cm21_unet_in_workspace_bytes(int B, int D);
+int cm21_unet_in_wgrad(const float* delta, const float* cond, const void* g, const void* y, float* dw, float* db,
+                       int B, int D, float delta_scale, void* work, size_t work_bytes, void* stream);
+
+/* Head: p = sigmoid(b + w . x) (x bf16 (B,D,D,D,16)) into prob (fp32, or
+ * NULL).  With a target (fp32 in [0,1]): loss = mean binary cross entropy
+ * (FP64, *loss), dx = dloss/dx (bf16, or NULL), dwb[0..15] = dloss/dw,
+ * dwb[16] = dloss/db. */
+size_t cm21_unet_head_workspace_bytes(void);
+int cm21_unet_head(const void* x, const float* w, const float* bias, const float* target, float* prob, void* dx,
+                   float* dwb, double* loss, int B, int D, void* work, size_t work_bytes, void* stream);
+
+/* bf16 copies of fp32 conv weights w [cout][27][cin]: wb same layout, wt the
+ * transposed, flipped kernel [cin][27][cout] (input gradient); either NULL. */
+int cm21_unet_pack(const float* w, void* wb, void* wt, int cin, int cout, void* stream);
+
+/* AdamW (decoupled weight decay) over n fp32 parameters, step >= 1. */
+int cm21_adamw(float* p, const float* g, float* m, float* v, size_t n, float lr, float beta1, float beta2, float eps,
+               float weight_decay, int step, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

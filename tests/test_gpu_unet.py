@@ -1,0 +1,301 @@
+"""GPU parity of the U-Net surrogate's layer kernels and of the whole
+forward/backward pass against the CPU oracle (oracle/unet.py, torch float64).
+
+Inputs of each layer test are bf16-exact, so a bf16 output may differ from
+the float64 oracle by its own rounding (half a bf16 ulp, 2^-8 relative) plus
+fp32 accumulation; fp32 weight gradients agree to 1e-3 of their largest
+entry.  The whole network is compared with the oracle that rounds at the same
+points as the GPU (bf16=True)."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import pkg
+from oracle import unet
+
+pytestmark = pytest.mark.gpu
+
+HALF_ULP = 2.0 ** -8
+
+
+def _bf(shape, scale=1.0, seed=0):
+    g = torch.Generator().manual_seed(seed)
+    return (torch.randn(shape, generator=g, dtype=torch.float64) * scale).to(torch.bfloat16).cuda()
+
+
+def _f(shape, scale=1.0, seed=0):
+    g = torch.Generator().manual_seed(seed)
+    return (torch.randn(shape, generator=g, dtype=torch.float64) * scale).to(torch.float32).cuda()
+
+
+def _d(t):
+    return t.detach().cpu().to(torch.float64)
+
+
+def _vp(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _fp(t):
+    return None if t is None else ctypes.cast(ctypes.c_void_p(t.data_ptr()), ctypes.POINTER(ctypes.c_float))
+
+
+def _close_bf16(got, ref, what):
+    got, ref = _d(got), ref.to(torch.float64)
+    tol = HALF_ULP * ref.abs() + 1e-5 * float(ref.abs().max()) + 1e-30
+    bad = ((got - ref).abs() > tol)
+    assert not bool(bad.any()), f"{what}: {int(bad.sum())} of {bad.numel()} outside, max err " \
+                                f"{float((got - ref).abs().max()):.3g}"
+
+
+def _close_f32(got, ref, what, rel=1e-3):
+    got, ref = _d(got), ref.to(torch.float64)
+    err = float((got - ref).abs().max())
+    assert err <= rel * float(ref.abs().max()) + 1e-12, f"{what}: max err {err:.3g} vs scale {float(ref.abs().max()):.3g}"
+
+
+def _conv_call(lib, B, D, cin, cout, x0, w, y0, x1=None, c0=None, mask=None, bias=None, relu=True, y1=None, o0=None,
+               per_sample=False):
+    c = lib.UnetConv(B=B, D=D, cin=cin, cout=cout, c0=c0 or cin, o0=o0 or cout, x0=_vp(x0), x1=_vp(x1),
+                     mask=_vp(mask), w=_vp(w), bias=_fp(bias), bias_per_sample=1 if per_sample else 0,
+                     relu=1 if relu else 0, y0=_vp(y0), y1=_vp(y1))
+    lib.call("cm21_unet_conv3_fwd", ctypes.byref(c), lib.stream_handle())
+
+
+SHAPES = [(16, 16, 16, False), (16, 32, 32, False), (32, 16, 64, True), (32, 32, 16, False), (32, 64, 16, False),
+          (64, 32, 32, True), (64, 64, 16, False), (64, 64, 64, False)]
+
+
+@pytest.mark.parametrize("cin,cout,D,split", SHAPES)
+def test_conv3_fwd(gpu, cin, cout, D, split):
+    lib = gpu
+    B = 2
+    x = _bf((B, D, D, D, cin), 0.7, 1)
+    w = _bf((cout, 27, cin), np.sqrt(2.0 / (27 * cin)), 2)
+    bias = _f((B, cout) if split else (cout,), 0.1, 3)
+    c0 = cin // 2 if split else cin
+    o0 = cout // 2 if split and cout >= 32 else cout
+    x0, x1 = x[..., :c0].contiguous(), (x[..., c0:].contiguous() if split else None)
+    y0 = torch.empty((B, D, D, D, o0), dtype=torch.bfloat16, device="cuda")
+    y1 = torch.empty((B, D, D, D, cout - o0), dtype=torch.bfloat16, device="cuda") if o0 < cout else None
+    print(f"conv fwd cin={cin} cout={cout} D={D} split={split}", flush=True)
+    _conv_call(lib, B, D, cin, cout, x0, w, y0, x1=x1, c0=c0, bias=bias, y1=y1, o0=o0, per_sample=split)
+    torch.cuda.synchronize()
+    ref = []
+    for b in range(B):
+        bb = _d(bias[b]) if split else _d(bias)
+        ref.append(unet.conv3(_d(x[b:b + 1]), _d(w), bb)[0])
+    ref = torch.stack(ref)
+    _close_bf16(y0, ref[..., :o0], "y0")
+    if y1 is not None:
+        _close_bf16(y1, ref[..., o0:], "y1")
+
+
+@pytest.mark.parametrize("cin,cout,D", [(16, 16, 32), (32, 16, 16), (64, 32, 16), (32, 64, 16)])
+def test_conv3_input_and_weight_gradients(gpu, cin, cout, D):
+    """Input gradient = the MFMA conv on the masked output gradient with the
+    packed transposed weights; weight/bias gradients from the wgrad kernel;
+    both against autograd of the oracle conv."""
+    lib = gpu
+    B = 2
+    x = _bf((B, D, D, D, cin), 0.7, 11)
+    w32 = _f((cout, 27, cin), np.sqrt(2.0 / (27 * cin)), 12)
+    g = _bf((B, D, D, D, cout), 0.5, 13)
+    m = _bf((B, D, D, D, cout), 1.0, 14)  # mixed signs: the ReLU mask
+    wb = torch.empty((cout, 27, cin), dtype=torch.bfloat16, device="cuda")
+    wt = torch.empty((cin, 27, cout), dtype=torch.bfloat16, device="cuda")
+    lib.call("cm21_unet_pack", _fp(w32), _vp(wb), _vp(wt), cin, cout, lib.stream_handle())
+    dx = torch.empty((B, D, D, D, cin), dtype=torch.bfloat16, device="cuda")
+    _conv_call(lib, B, D, cout, cin, g, wt, dx, mask=m, relu=False)
+    dw = torch.empty((cout, 27, cin), dtype=torch.float32, device="cuda")
+    db = torch.empty(cout, dtype=torch.float32, device="cuda")
+    work = torch.empty(lib.query("cm21_unet_wgrad_workspace_bytes", B, D, cin, cout), dtype=torch.uint8,
+                       device="cuda")
+    wg = lib.UnetWgrad(B=B, D=D, cin=cin, cout=cout, c0=cin, x0=_vp(x), g=_vp(g), mask=_vp(m), dw=_fp(dw),
+                       db=_fp(db), work=_vp(work), work_bytes=work.numel())
+    lib.call("cm21_unet_conv3_wgrad", ctypes.byref(wg), lib.stream_handle())
+    torch.cuda.synchronize()
+    assert torch.equal(wb.cpu(), w32.cpu().to(torch.bfloat16))
+    xr = _d(x).requires_grad_(True)
+    wr = _d(wb).requires_grad_(True)
+    br = torch.zeros(cout, dtype=torch.float64, requires_grad=True)
+    gm = _d(g) * (_d(m) > 0)
+    (unet.conv3(xr, wr, br, relu=False) * gm).sum().backward()
+    _close_bf16(dx, xr.grad, "dx")
+    # the weight gradient uses the bf16 input exactly as stored
+    _close_f32(dw, wr.grad, "dw")
+    _close_f32(db, br.grad, "db")
+
+
+def test_pool_and_up(gpu):
+    lib = gpu
+    B, D = 2, 32
+    x = _bf((B, D, D, D, 32), 1.0, 21)
+    y = torch.empty((B, D // 2, D // 2, D // 2, 32), dtype=torch.bfloat16, device="cuda")
+    lib.call("cm21_unet_pool_fwd", _vp(x), _vp(y), B, D, 32, lib.stream_handle())
+    dy = _bf((B, D // 2, D // 2, D // 2, 32), 1.0, 22)
+    dx = _bf((B, D, D, D, 32), 1.0, 23)
+    dx0 = _d(dx)
+    lib.call("cm21_unet_pool_bwd", _vp(dy), _vp(dx), B, D, 32, 1, lib.stream_handle())
+    torch.cuda.synchronize()
+    _close_bf16(y, unet.pool(_d(x)), "pool")
+    up_dy = _d(dy).repeat_interleave(2, 1).repeat_interleave(2, 2).repeat_interleave(2, 3) / 8
+    _close_bf16(dx, dx0 + up_dy, "pool bwd (accumulate)")
+    for cin, cout, Din in ((64, 32, 16), (32, 16, 32)):
+        xi = _bf((B, Din, Din, Din, cin), 0.8, 24)
+        w = _f((cout, 8, cin), np.sqrt(1.0 / cin), 25)
+        bia = _f((cout,), 0.1, 26)
+        yo = torch.empty((B, 2 * Din, 2 * Din, 2 * Din, cout), dtype=torch.bfloat16, device="cuda")
+        lib.call("cm21_unet_up_fwd", _vp(xi), _fp(w), _fp(bia), _vp(yo), B, Din, cin, cout, lib.stream_handle())
+        g = _bf((B, 2 * Din, 2 * Din, 2 * Din, cout), 0.5, 27)
+        gx = torch.empty_like(xi)
+        dw = torch.empty_like(w)
+        db = torch.empty_like(bia)
+        work = torch.empty(lib.query("cm21_unet_up_workspace_bytes", B, Din, cin, cout), dtype=torch.uint8,
+                           device="cuda")
+        lib.call("cm21_unet_up_bwd", _vp(xi), _vp(g), _fp(w), _vp(gx), _fp(dw), _fp(db), B, Din, cin, cout,
+                 _vp(work), work.numel(), lib.stream_handle())
+        torch.cuda.synchronize()
+        xr = _d(xi).requires_grad_(True)
+        wr = _d(w).requires_grad_(True)
+        br = _d(bia).requires_grad_(True)
+        out = unet.up(xr, wr, br)
+        _close_bf16(yo, out.detach(), f"up fwd {cin}->{cout}")
+        (out * _d(g)).sum().backward()
+        _close_bf16(gx, xr.grad, f"up dx {cin}->{cout}")
+        _close_f32(dw, wr.grad, f"up dw {cin}->{cout}")
+        _close_f32(db, br.grad, f"up db {cin}->{cout}")
+
+
+def test_input_layer(gpu):
+    lib = gpu
+    B, D = 2, 32
+    delta = _f((B, D, D, D), 0.6, 31)
+    cond = torch.tensor([[0.3, -0.7], [-1.0, 0.9]], dtype=torch.float32, device="cuda")
+    w = _f((16, 27, 3), np.sqrt(2.0 / 81), 32)
+    bia = _f((16,), 0.1, 33)
+    beff = torch.empty((B, 16), dtype=torch.float32, device="cuda")
+    y = torch.empty((B, D, D, D, 16), dtype=torch.bfloat16, device="cuda")
+    scale = 1 / 0.6
+    lib.call("cm21_unet_in_fwd", _fp(delta), _fp(cond), _fp(w), _fp(bia), _fp(beff), _vp(y), B, D,
+             ctypes.c_float(scale), lib.stream_handle())
+    g = _bf((B, D, D, D, 16), 0.5, 34)
+    dw = torch.empty_like(w)
+    db = torch.empty_like(bia)
+    work = torch.empty(lib.query("cm21_unet_in_workspace_bytes", B, D), dtype=torch.uint8, device="cuda")
+    lib.call("cm21_unet_in_wgrad", _fp(delta), _fp(cond), _vp(g), _vp(y), _fp(dw), _fp(db), B, D,
+             ctypes.c_float(scale), _vp(work), work.numel(), lib.stream_handle())
+    torch.cuda.synchronize()
+    dr = _d(delta)
+    cr = _d(cond)
+    x = torch.stack([dr * scale, cr[:, 0].reshape(B, 1, 1, 1).expand(B, D, D, D),
+                     cr[:, 1].reshape(B, 1, 1, 1).expand(B, D, D, D)], -1)
+    wr = _d(w).requires_grad_(True)
+    br = _d(bia).requires_grad_(True)
+    out = unet.conv3(x, wr, br)
+    _close_bf16(y, out.detach(), "input layer")
+    (out * (_d(g) * (_d(y) > 0))).sum().backward()
+    _close_f32(dw, wr.grad, "input dw")
+    _close_f32(db, br.grad, "input db")
+
+
+def test_head_loss_and_gradients(gpu):
+    lib = gpu
+    B, D = 2, 32
+    x = _bf((B, D, D, D, 16), 0.8, 41)
+    w = _f((17,), 0.4, 42)
+    target = torch.rand((B, D, D, D), generator=torch.Generator().manual_seed(43)).float().cuda()
+    prob = torch.empty((B, D, D, D), dtype=torch.float32, device="cuda")
+    dx = torch.empty_like(x)
+    dwb = torch.empty(17, dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    work = torch.empty(lib.query("cm21_unet_head_workspace_bytes"), dtype=torch.uint8, device="cuda")
+    lib.call("cm21_unet_head", _vp(x), _fp(w[:16]), _fp(w[16:]), _fp(target), _fp(prob), _vp(dx), _fp(dwb),
+             ctypes.cast(ctypes.c_void_p(loss.data_ptr()), ctypes.POINTER(ctypes.c_double)), B, D, _vp(work),
+             work.numel(), lib.stream_handle())
+    torch.cuda.synchronize()
+    xr = _d(x).requires_grad_(True)
+    wr = _d(w).requires_grad_(True)
+    lg = (xr * wr[:16]).sum(-1) + wr[16]
+    ref = unet.bce(lg, _d(target))
+    ref.backward()
+    assert float(loss) == pytest.approx(float(ref), rel=1e-6)
+    _close_f32(prob, torch.sigmoid(lg).detach(), "prob", rel=1e-6)
+    _close_bf16(dx, xr.grad, "head dx")
+    _close_f32(dwb, wr.grad, "head dw/db")
+
+
+def test_adamw(gpu):
+    lib = gpu
+    n = 1000
+    rng = np.random.default_rng(5)
+    p0, g, m0, v0 = (rng.standard_normal(n).astype(np.float32) for _ in range(4))
+    v0 = np.abs(v0)
+    p, m, v = (torch.from_numpy(a.copy()).cuda() for a in (p0, m0, v0))
+    gt = torch.from_numpy(g).cuda()
+    lr, b1, b2, eps, wd, step = 1e-3, 0.9, 0.999, 1e-8, 0.01, 7
+    lib.call("cm21_adamw", _fp(p), _fp(gt), _fp(m), _fp(v), n, ctypes.c_float(lr), ctypes.c_float(b1),
+             ctypes.c_float(b2), ctypes.c_float(eps), ctypes.c_float(wd), step, lib.stream_handle())
+    torch.cuda.synchronize()
+    mm = b1 * m0.astype(np.float64) + (1 - b1) * g
+    vv = b2 * v0.astype(np.float64) + (1 - b2) * g.astype(np.float64) ** 2
+    pp = p0 - lr * ((mm / (1 - b1 ** step)) / (np.sqrt(vv / (1 - b2 ** step)) + eps) + wd * p0)
+    np.testing.assert_allclose(m.cpu().numpy(), mm, rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(v.cpu().numpy(), vv, rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(p.cpu().numpy(), pp, rtol=1e-5, atol=1e-6)
+
+
+def _net_inputs(D, B, seed):
+    rng = np.random.default_rng(seed)
+    delta = (rng.standard_normal((B, D, D, D)) * 0.6).astype(np.float32)
+    cond = rng.uniform(-1, 1, (B, 2)).astype(np.float32)
+    target = rng.uniform(0, 1, (B, D, D, D)).astype(np.float32)
+    target[:, : D // 2] = (target[:, : D // 2] > 0.5)
+    return delta, cond, target
+
+
+def test_network_forward_and_gradients_match_oracle(gpu):
+    sur = pkg("surrogate")
+    D, B = 64, 2
+    P = sur.init_params(1)
+    rng = np.random.default_rng(9)
+    for k in P:
+        if k.endswith(".b"):
+            P[k] = rng.standard_normal(P[k].shape) * 0.05
+    model = sur.UNet3D(D=D, batch=B, params=P)
+    # the GPU keeps fp32 masters: compare against the oracle on the same fp32 values
+    P32 = {k: v.astype(np.float32).astype(np.float64) for k, v in P.items()}
+    delta, cond, target = _net_inputs(D, B, 10)
+    prob = model.forward(torch.from_numpy(delta).cuda(), cond).cpu().numpy()
+    loss = float(model.backward(torch.from_numpy(target).cuda()))
+    torch.cuda.synchronize()
+    ref_prob = unet.predict(P32, delta, cond, sur.DELTA_SCALE, bf16=True)
+    err = np.abs(prob - ref_prob)
+    print(f"prob err mean {err.mean():.2e} max {err.max():.2e}", flush=True)
+    assert err.mean() < 2e-3 and err.max() < 5e-2
+    ref_loss, ref_g = unet.loss_and_grads(P32, delta, cond, target, sur.DELTA_SCALE, bf16=True)
+    assert loss == pytest.approx(ref_loss, rel=2e-3)
+    for name, gref in ref_g.items():
+        gg = model.g[name].detach().cpu().double().numpy()
+        rel = np.linalg.norm(gg - gref) / max(np.linalg.norm(gref), 1e-30)
+        cos = float(np.dot(gg.ravel(), gref.ravel()) / max(np.linalg.norm(gg) * np.linalg.norm(gref), 1e-30))
+        print(f"{name}: rel L2 {rel:.2e} cos {cos:.5f}", flush=True)
+        assert rel < 0.1 and cos > 0.99, name
+
+
+def test_training_reduces_loss_on_plain_targets(gpu):
+    """A few AdamW steps on boxes labelled by the plain method lower the loss."""
+    sur = pkg("surrogate")
+    specs = sur.surrogate_specs(range(1000, 1008), D=64, box_len=100.0)
+    boxes = sur.BoxSet(specs)
+    xhi = boxes.x_HI()
+    print("x_HI of the boxes", np.round(xhi, 3), flush=True)
+    assert np.any((xhi > 0.05) & (xhi < 0.95))
+    model = sur.UNet3D(D=64, batch=2, seed=0)
+    losses = sur.train(model, boxes, steps=40, lr=3e-3, log_every=1)
+    first = np.mean([l for _, l in losses[:5]])
+    last = np.mean([l for _, l in losses[-5:]])
+    print(f"loss {first:.4f} -> {last:.4f}", flush=True)
+    assert last < 0.8 * first

@@ -35,7 +35,7 @@ def cdll():
 def test_header_parse_finds_all_entry_points():
     names = header_functions()
     assert "cm21_find_bubbles" in names and "cm21_power_spectrum" in names
-    assert "cm21_delta_T" in names and len(names) == 28
+    assert "cm21_delta_T" in names and len(names) == 43
 
 
 def test_library_exports_every_declared_symbol(cdll):
@@ -56,21 +56,25 @@ def test_nm_dynamic_exports(cdll):
     assert set(header_functions()) <= exported
 
 
-def test_bubble_params_layout_matches_c(tmp_path):
-    """ctypes BubbleParams has the C struct's size and field offsets (gcc on the header)."""
+@pytest.mark.parametrize("cname,pyname", [("cm21_bubble_params", "BubbleParams"), ("cm21_batch_box", "BatchBox"),
+                                           ("cm21_batch_group", "BatchGroup"), ("cm21_unet_conv", "UnetConv"),
+                                           ("cm21_unet_wgrad", "UnetWgrad")])
+def test_struct_layout_matches_c(tmp_path, cname, pyname):
+    """Each ctypes structure has the C struct's size and field offsets (gcc on the header)."""
     lib_mod = pkg("_lib")
-    fields = [f for f, _ in lib_mod.BubbleParams._fields_]
+    cls = getattr(lib_mod, pyname)
+    fields = [f for f, _ in cls._fields_]
     src = tmp_path / "layout.c"
-    body = "".join(f'    printf("{f} %zu\\n", offsetof(cm21_bubble_params, {f}));\n' for f in fields)
+    body = "".join(f'    printf("{f} %zu\\n", offsetof({cname}, {f}));\n' for f in fields)
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "cm21.h"\nint main(void){\n'
-                   '    printf("sizeof %zu\\n", sizeof(cm21_bubble_params));\n' + body + "    return 0;\n}\n")
+                   f'    printf("sizeof %zu\\n", sizeof({cname}));\n' + body + "    return 0;\n}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
                                                         check=True).stdout.splitlines())
-    assert int(got["sizeof"]) == ctypes.sizeof(lib_mod.BubbleParams)
+    assert int(got["sizeof"]) == ctypes.sizeof(cls)
     for f in fields:
-        assert int(got[f]) == getattr(lib_mod.BubbleParams, f).offset, f
+        assert int(got[f]) == getattr(cls, f).offset, f
 
 
 def test_abi_version_and_argument_errors(cdll):
