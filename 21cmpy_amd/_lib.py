@@ -138,9 +138,10 @@ SIGNATURES = {
     "cm21_unet_conv3_wgrad": (I, [ctypes.POINTER(UnetWgrad), VP]),
     "cm21_unet_pool_fwd": (I, [VP, VP, I, I, I, VP]),
     "cm21_unet_pool_bwd": (I, [VP, VP, I, I, I, I, VP]),
-    "cm21_unet_up_fwd": (I, [VP, c_float_p, c_float_p, VP, I, I, I, I, VP]),
+    "cm21_unet_pack_up": (I, [c_float_p, VP, VP, I, I, VP]),
+    "cm21_unet_up_fwd": (I, [VP, VP, c_float_p, VP, I, I, I, I, VP]),
     "cm21_unet_up_workspace_bytes": (SZ, [I, I, I, I]),
-    "cm21_unet_up_bwd": (I, [VP, VP, c_float_p, VP, c_float_p, c_float_p, I, I, I, I, VP, SZ, VP]),
+    "cm21_unet_up_bwd": (I, [VP, VP, VP, VP, c_float_p, c_float_p, I, I, I, I, VP, SZ, VP]),
     "cm21_unet_in_fwd": (I, [c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, VP, I, I, F, VP]),
     "cm21_unet_in_workspace_bytes": (SZ, [I, I]),
     "cm21_unet_in_wgrad": (I, [c_float_p, c_float_p, VP, VP, c_float_p, c_float_p, I, I, F, VP, SZ, VP]),

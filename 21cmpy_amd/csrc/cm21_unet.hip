@@ -120,28 +120,41 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
 #pragma unroll
         for (int n = 0; n < NT; ++n) acc[m][n] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     const u32x4 zero = {0u, 0u, 0u, 0u};
+    // K is padded to a multiple of 32 with zeros; addresses of the padding are
+    // clamped to tap 0 so that every access stays inside the tile and the
+    // weight array.  The weight fragments of slice kc+1 are fetched while the
+    // MFMAs of slice kc run.
+    auto load_b = [&](int kc, bf16x8 (&bw)[NT]) {
+        const int kraw = kc * 32 + grp * 8;
+        const bool valid = kraw < K;
+        const int k0 = valid ? kraw : 0;
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+            bw[n] = __builtin_bit_cast(bf16x8, valid ? *(const u32x4*)(a.w + (size_t)(n * 16 + row) * K + k0) : zero);
+    };
+    bf16x8 bcur[NT], bnext[NT];
+    load_b(0, bcur);
     for (int kc = 0; kc < KC; ++kc) {
-        // K is padded to a multiple of 32 with zeros; addresses of the padding
-        // are clamped to tap 0 so that every access stays inside the tile and
-        // the weight array
+        if (kc + 1 < KC) load_b(kc + 1, bnext);
         const int kraw = kc * 32 + grp * 8;
         const bool valid = kraw < K;
         const int k0 = valid ? kraw : 0;
         const int tap = k0 / CIN, ci = k0 % CIN;
-        bf16x8 bw[NT];
-#pragma unroll
-        for (int n = 0; n < NT; ++n)
-            bw[n] = __builtin_bit_cast(bf16x8, valid ? *(const u32x4*)(a.w + (size_t)(n * 16 + row) * K + k0) : zero);
         const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+        bf16x8 av[MW];
 #pragma unroll
         for (int m = 0; m < MW; ++m) {
             const int mt = wave * MW + m;
             const int lz = mt / TY, ly = mt % TY;
             const int off = (((lz + dz) * HY + (ly + dy)) * HX + (row + dx)) * CP + ci;
-            const bf16x8 av = __builtin_bit_cast(bf16x8, valid ? *(const u32x4*)(tile + off) : zero);
-#pragma unroll
-            for (int n = 0; n < NT; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw[n], acc[m][n], 0, 0, 0);
+            av[m] = __builtin_bit_cast(bf16x8, valid ? *(const u32x4*)(tile + off) : zero);
         }
+#pragma unroll
+        for (int m = 0; m < MW; ++m)
+#pragma unroll
+            for (int n = 0; n < NT; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bcur[n], acc[m][n], 0, 0, 0);
+#pragma unroll
+        for (int n = 0; n < NT; ++n) bcur[n] = bnext[n];
     }
     const int o1 = COUT - a.o0;
 #pragma unroll
@@ -291,31 +304,57 @@ __global__ void k_sum_blocks(const float* __restrict__ partial, int nblk, int n,
     out[i] = s;
 }
 
-// Per-sample channel sums of a (masked) gradient: partial[b][blk][c].
-__global__ void k_channel_partials(const bf16_t* __restrict__ g, const bf16_t* __restrict__ mask, int C,
-                                   size_t vox_per_sample, int blocks_per_sample, float* __restrict__ partial) {
-    __shared__ float red[256];
+// Per-sample channel sums of a (masked) gradient over a chunk of one
+// sample's voxels per block: partial[b][blk][c].  Each thread reads 16-byte
+// vectors of 8 channels; lanes with the same vector slot are added in a
+// fixed order at the end.
+__global__ __launch_bounds__(256) void k_channel_partials(const bf16_t* __restrict__ g,
+                                                          const bf16_t* __restrict__ mask, int C,
+                                                          size_t vox_per_sample, float* __restrict__ partial) {
+    __shared__ float red[256 * 8];
     const int b = blockIdx.y;
-    const int c = threadIdx.x % C;  // C divides 256
-    const int lanes = 256 / C;
-    const int l = threadIdx.x / C;
-    float s = 0.0f;
-    const size_t per = (vox_per_sample + blocks_per_sample - 1) / blocks_per_sample;
+    const int vc = C / 8;  // 16-byte vectors per voxel (C in {16, 32, 64})
+    const int t = threadIdx.x;
+    const int cv = t % vc, lanes = 256 / vc, l = t / vc;
+    const size_t per = (vox_per_sample + gridDim.x - 1) / gridDim.x;
     const size_t v0 = (size_t)blockIdx.x * per;
     const size_t v1 = v0 + per < vox_per_sample ? v0 + per : vox_per_sample;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (size_t v = v0 + l; v < v1; v += lanes) {
-        const size_t idx = ((size_t)b * vox_per_sample + v) * C + c;
-        float gv = bf2f(g[idx]);
-        if (mask && !pos_bf(mask[idx])) gv = 0.0f;
-        s += gv;
+        const size_t idx = ((size_t)b * vox_per_sample + v) * C + cv * 8;
+        u32x4 q = *(const u32x4*)(g + idx);
+        if (mask) q = mask8(q, *(const u32x4*)(mask + idx));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            s[2 * k] += __uint_as_float(q[k] << 16);
+            s[2 * k + 1] += __uint_as_float(q[k] & 0xffff0000u);
+        }
     }
-    red[threadIdx.x] = s;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) red[t * 8 + k] = s[k];
     __syncthreads();
-    if (threadIdx.x < C) {
-        float t = 0.0f;
-        for (int q = 0; q < lanes; ++q) t += red[q * C + threadIdx.x];
-        partial[((size_t)b * blocks_per_sample + blockIdx.x) * C + threadIdx.x] = t;
+    if (t < C) {
+        const int c8 = t / 8, k = t % 8;
+        float acc = 0.0f;
+        for (int q = 0; q < lanes; ++q) acc += red[(q * vc + c8) * 8 + k];
+        partial[((size_t)b * gridDim.x + blockIdx.x) * C + t] = acc;
     }
+}
+
+// per_sample[b][c] = sum_blk partial[b][blk][c]; total[c] = sum_b per_sample[b][c]
+// (either output may be NULL).
+__global__ void k_channel_final(const float* __restrict__ partial, int B, int nblk, int C,
+                                float* __restrict__ per_sample, float* __restrict__ total) {
+    const int c = threadIdx.x;
+    if (c >= C) return;
+    float tot = 0.0f;
+    for (int b = 0; b < B; ++b) {
+        float s = 0.0f;
+        for (int k = 0; k < nblk; ++k) s += partial[((size_t)b * nblk + k) * C + c];
+        if (per_sample) per_sample[b * C + c] = s;
+        tot += s;
+    }
+    if (total) total[c] = tot;
 }
 
 // ------------------------------------------------------------------ pooling
@@ -354,61 +393,95 @@ __global__ void k_pool_bwd(const bf16_t* __restrict__ dy, bf16_t* __restrict__ d
 
 // ------------------------------------------------------------------ 2^3 stride-2 transposed conv
 
-// y[2v+s][co] = bias[co] + sum_ci w[co][s][ci] x[v][ci]; one thread per (input voxel, s).
+// Matrix-core forms of the transposed convolution.  Forward: for 16
+// consecutive input voxels (M) and n = s * COUT + co (N), K = CIN:
+// y[2v+s][co] = bias[co] + sum_ci wn[n][ci] x[v][ci], with wn bf16 [8*COUT][CIN].
+// Input gradient: K = (s, co), N = ci: dx[v][ci] = sum_k wk[ci][k] dy[2v+s][co],
+// with wk bf16 [CIN][8*COUT].  A and B fragments are 16-byte loads of 8
+// consecutive channels; each wave walks its 16-voxel row segments.
 template <int CIN, int COUT>
-__global__ __launch_bounds__(256) void k_up_fwd(const bf16_t* __restrict__ x, const float* __restrict__ w,
-                                                const float* __restrict__ bias, bf16_t* __restrict__ y, int B, int D) {
-    __shared__ float ws[COUT * 8 * CIN];
-    for (int i = threadIdx.x; i < COUT * 8 * CIN; i += 256) ws[i] = w[i];
-    __syncthreads();
-    const size_t nv = (size_t)B * D * D * D;
-    const size_t item = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (item >= nv * 8) return;
-    const int s = item % 8;
-    const size_t v = item / 8;
-    float xv[CIN];
+__global__ __launch_bounds__(256) void k_up_fwd_mfma(const bf16_t* __restrict__ x, const bf16_t* __restrict__ wn,
+                                                     const float* __restrict__ bias, bf16_t* __restrict__ y, int B,
+                                                     int D) {
+    constexpr int KC = CIN / 32, NT = 8 * COUT / 16;
+    static_assert(CIN % 32 == 0 && COUT % 16 == 0, "channel counts");
+    const int lane = threadIdx.x & 63, row = lane & 15, grp = lane >> 4;
+    const size_t nseg = (size_t)B * D * D * D / 16;
+    const size_t seg = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (seg >= nseg) return;  // whole wave
+    const size_t v0 = seg * 16;
+    bf16x8 av[KC];
 #pragma unroll
-    for (int c = 0; c < CIN; ++c) xv[c] = bf2f(x[v * CIN + c]);
-    const int xx = v % D, yy = (v / D) % D, zz = (v / ((size_t)D * D)) % D;
-    const int b = v / ((size_t)D * D * D);
+    for (int k = 0; k < KC; ++k) av[k] = __builtin_bit_cast(bf16x8, *(const u32x4*)(x + (v0 + row) * CIN + k * 32 + grp * 8));
     const int D2 = 2 * D;
-    const size_t u = (((size_t)b * D2 + 2 * zz + (s >> 2)) * D2 + 2 * yy + ((s >> 1) & 1)) * D2 + 2 * xx + (s & 1);
-    for (int co = 0; co < COUT; ++co) {
-        float acc = bias[co];
-        const float* wp = ws + (co * 8 + s) * CIN;
+#pragma unroll 2
+    for (int j = 0; j < NT; ++j) {
+        floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int c = 0; c < CIN; ++c) acc = fmaf(wp[c], xv[c], acc);
-        y[u * COUT + co] = f2bf(acc);
+        for (int k = 0; k < KC; ++k) {
+            const bf16x8 bw = __builtin_bit_cast(bf16x8, *(const u32x4*)(wn + (size_t)(j * 16 + row) * CIN + k * 32 + grp * 8));
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[k], bw, acc, 0, 0, 0);
+        }
+        const int n = j * 16 + row;
+        const int s = n / COUT, co = n % COUT;
+        const float bv = bias[co];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const size_t v = v0 + grp * 4 + i;
+            const int xx = v % D, yy = (v / D) % D, zz = (v / ((size_t)D * D)) % D;
+            const int b = v / ((size_t)D * D * D);
+            const size_t u = (((size_t)b * D2 + 2 * zz + (s >> 2)) * D2 + 2 * yy + ((s >> 1) & 1)) * D2 + 2 * xx + (s & 1);
+            y[u * COUT + co] = f2bf(acc[i] + bv);
+        }
     }
 }
 
-// dx[v][ci] = sum_{s,co} w[co][s][ci] dy[2v+s][co]; one thread per input voxel.
 template <int CIN, int COUT>
-__global__ __launch_bounds__(256) void k_up_bwd_data(const bf16_t* __restrict__ dy, const float* __restrict__ w,
+__global__ __launch_bounds__(256) void k_up_bwd_mfma(const bf16_t* __restrict__ dy, const bf16_t* __restrict__ wk,
                                                      bf16_t* __restrict__ dx, int B, int D) {
-    __shared__ float ws[COUT * 8 * CIN];
-    for (int i = threadIdx.x; i < COUT * 8 * CIN; i += 256) ws[i] = w[i];
-    __syncthreads();
-    const size_t nv = (size_t)B * D * D * D;
-    const size_t v = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (v >= nv) return;
+    constexpr int K = 8 * COUT, KC = K / 32, NT = CIN / 16;
+    static_assert(COUT % 4 == 0 && CIN % 16 == 0 && K % 32 == 0, "channel counts");
+    const int lane = threadIdx.x & 63, row = lane & 15, grp = lane >> 4;
+    const size_t nseg = (size_t)B * D * D * D / 16;
+    const size_t seg = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (seg >= nseg) return;  // whole wave
+    const size_t v = seg * 16 + row;  // this lane's A row (input voxel)
     const int xx = v % D, yy = (v / D) % D, zz = (v / ((size_t)D * D)) % D;
     const int b = v / ((size_t)D * D * D);
     const int D2 = 2 * D;
-    float acc[CIN];
+    floatx4 acc[NT];
 #pragma unroll
-    for (int c = 0; c < CIN; ++c) acc[c] = 0.0f;
-    for (int s = 0; s < 8; ++s) {
+    for (int n = 0; n < NT; ++n) acc[n] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+    for (int kc = 0; kc < KC; ++kc) {
+        const int k0 = kc * 32 + grp * 8;  // 8 consecutive (s, co) with one s
+        const int s = k0 / COUT, co = k0 % COUT;
         const size_t u = (((size_t)b * D2 + 2 * zz + (s >> 2)) * D2 + 2 * yy + ((s >> 1) & 1)) * D2 + 2 * xx + (s & 1);
-        for (int co = 0; co < COUT; ++co) {
-            const float g = bf2f(dy[u * COUT + co]);
-            const float* wp = ws + (co * 8 + s) * CIN;
+        const bf16x8 a = __builtin_bit_cast(bf16x8, *(const u32x4*)(dy + u * COUT + co));
 #pragma unroll
-            for (int c = 0; c < CIN; ++c) acc[c] = fmaf(wp[c], g, acc[c]);
+        for (int n = 0; n < NT; ++n) {
+            const bf16x8 bw = __builtin_bit_cast(bf16x8, *(const u32x4*)(wk + (size_t)(n * 16 + row) * K + k0));
+            acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc[n], 0, 0, 0);
         }
     }
 #pragma unroll
-    for (int c = 0; c < CIN; ++c) dx[v * CIN + c] = f2bf(acc[c]);
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dx[(seg * 16 + grp * 4 + i) * CIN + n * 16 + row] = f2bf(acc[n][i]);
+}
+
+// bf16 copies of transposed-conv weights w [cout][8][cin]:
+// wn[s*cout+co][ci] (forward) and wk[ci][s*cout+co] (input gradient).
+__global__ void k_pack_up(const float* __restrict__ w, bf16_t* __restrict__ wn, bf16_t* __restrict__ wk, int cin,
+                          int cout) {
+    const int n = cout * 8 * cin;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ci = i % cin, s = (i / cin) % 8, co = i / (8 * cin);
+    const bf16_t h = f2bf(w[i]);
+    const int k = s * cout + co;
+    wn[(size_t)k * cin + ci] = h;
+    wk[(size_t)ci * 8 * cout + k] = h;
 }
 
 // dW[co][s][ci] = sum_v dy[2v+s][co] x[v][ci]: block partials over voxel
@@ -439,11 +512,13 @@ __global__ __launch_bounds__(256) void k_up_wgrad(const bf16_t* __restrict__ x, 
             const int b = v / ((size_t)D * D * D);
             const size_t u =
                 (((size_t)b * D2 + 2 * zz + (s >> 2)) * D2 + 2 * yy + ((s >> 1) & 1)) * D2 + 2 * xx + (s & 1);
-            float gv[4], xv[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) gv[i] = bf2f(dy[u * COUT + co0 + i]);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) xv[j] = bf2f(x[v * CIN + ci0 + j]);
+            // 4 consecutive bf16 = one 8-byte load each
+            const uint2 gq = *(const uint2*)(dy + u * COUT + co0);
+            const uint2 xq = *(const uint2*)(x + v * CIN + ci0);
+            const float gv[4] = {__uint_as_float(gq.x << 16), __uint_as_float(gq.x & 0xffff0000u),
+                                 __uint_as_float(gq.y << 16), __uint_as_float(gq.y & 0xffff0000u)};
+            const float xv[4] = {__uint_as_float(xq.x << 16), __uint_as_float(xq.x & 0xffff0000u),
+                                 __uint_as_float(xq.y << 16), __uint_as_float(xq.y & 0xffff0000u)};
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -506,11 +581,24 @@ __global__ __launch_bounds__(256) void k_in_fwd(const float* __restrict__ delta,
         const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
         d[t] = scale * delta[(((size_t)b * D + wrap(zz + dz, D)) * D + wrap(yy + dy, D)) * D + wrap(xx + dx, D)];
     }
-    for (int co = 0; co < COUT; ++co) {
-        float acc = beff[b * COUT + co];
+    static_assert(COUT % 8 == 0, "16-byte output vectors");
 #pragma unroll
-        for (int t = 0; t < 27; ++t) acc = fmaf(ws[co * 27 + t], d[t], acc);
-        y[v * COUT + co] = f2bf(fmaxf(acc, 0.0f));
+    for (int c8 = 0; c8 < COUT / 8; ++c8) {
+        u32x4 out;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            unsigned pair = 0;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int co = c8 * 8 + 2 * k + h;
+                float acc = beff[b * COUT + co];
+#pragma unroll
+                for (int t = 0; t < 27; ++t) acc = fmaf(ws[co * 27 + t], d[t], acc);
+                pair |= (unsigned)f2bf(fmaxf(acc, 0.0f)) << (16 * h);
+            }
+            out[k] = pair;
+        }
+        *(u32x4*)(y + v * COUT + c8 * 8) = out;
     }
 }
 
@@ -755,7 +843,17 @@ const WgradShape* wgrad_shape(int cin, int cout) {
 }
 
 constexpr int kWgradBlocks = 256;   // voxel-slab blocks of the weight gradient
-constexpr int kChanBlocks = 64;     // per-sample blocks of the channel sums
+
+constexpr int kChanBlocks = 128;    // per-sample blocks of the channel sums
+
+int channel_sums(const void* g, const void* mask, int C, size_t vps, int B, float* cpart, float* total,
+                 float* per_sample, hipStream_t st) {
+    k_channel_partials<<<dim3(kChanBlocks, B), 256, 0, st>>>((const bf16_t*)g, (const bf16_t*)mask, C, vps, cpart);
+    CM21_LAUNCH_CHECK("k_channel_partials");
+    k_channel_final<<<1, 64, 0, st>>>(cpart, B, kChanBlocks, C, per_sample, total);
+    CM21_LAUNCH_CHECK("k_channel_final");
+    return CM21_OK;
+}
 constexpr int kHeadBlocks = 1024;
 constexpr int kUpBlocksMax = 256;
 
@@ -836,14 +934,7 @@ extern "C" int cm21_unet_conv3_wgrad(const cm21_unet_wgrad* w, void* stream) {
     const int n = w->cout * 27 * w->cin;
     k_sum_blocks<<<(n + 255) / 256, 256, 0, st>>>(partial, nblk, n, w->dw);
     CM21_LAUNCH_CHECK("k_sum_blocks");
-    if (w->db) {
-        const size_t vps = (size_t)w->D * w->D * w->D;
-        k_channel_partials<<<dim3(kChanBlocks, w->B), 256, 0, st>>>((const bf16_t*)w->g, (const bf16_t*)w->mask,
-                                                                     w->cout, vps, kChanBlocks, cpart);
-        CM21_LAUNCH_CHECK("k_channel_partials");
-        k_sum_blocks<<<1, 256, 0, st>>>(cpart, w->B * kChanBlocks, w->cout, w->db);
-        CM21_LAUNCH_CHECK("k_sum_blocks");
-    }
+    if (w->db) CM21_TRY(channel_sums(w->g, w->mask, w->cout, (size_t)w->D * w->D * w->D, w->B, cpart, w->db, nullptr, st));
     return CM21_OK;
 }
 
@@ -866,19 +957,21 @@ extern "C" int cm21_unet_pool_bwd(const void* dy, void* dx, int B, int D, int C,
 
 namespace {
 template <int CIN, int COUT>
-int up_fwd_t(const void* x, const float* w, const float* b, void* y, int B, int D, hipStream_t st) {
-    const size_t items = (size_t)B * D * D * D * 8;
-    k_up_fwd<CIN, COUT><<<(unsigned)((items + 255) / 256), 256, 0, st>>>((const bf16_t*)x, w, b, (bf16_t*)y, B, D);
-    CM21_LAUNCH_CHECK("k_up_fwd");
+int up_fwd_t(const void* x, const void* wn, const float* b, void* y, int B, int D, hipStream_t st) {
+    const size_t nseg = (size_t)B * D * D * D / 16;
+    k_up_fwd_mfma<CIN, COUT><<<(unsigned)((nseg + 3) / 4), 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)wn, b,
+                                                                        (bf16_t*)y, B, D);
+    CM21_LAUNCH_CHECK("k_up_fwd_mfma");
     return CM21_OK;
 }
 template <int CIN, int COUT>
-int up_bwd_t(const void* x, const void* dy, const float* w, void* dx, float* dw, float* db, int B, int D, float* part,
+int up_bwd_t(const void* x, const void* dy, const void* wk, void* dx, float* dw, float* db, int B, int D, float* part,
              float* cpart, hipStream_t st) {
     const size_t nv = (size_t)B * D * D * D;
     if (dx) {
-        k_up_bwd_data<CIN, COUT><<<(unsigned)((nv + 255) / 256), 256, 0, st>>>((const bf16_t*)dy, w, (bf16_t*)dx, B, D);
-        CM21_LAUNCH_CHECK("k_up_bwd_data");
+        k_up_bwd_mfma<CIN, COUT><<<(unsigned)((nv / 16 + 3) / 4), 256, 0, st>>>((const bf16_t*)dy, (const bf16_t*)wk,
+                                                                               (bf16_t*)dx, B, D);
+        CM21_LAUNCH_CHECK("k_up_bwd_mfma");
     }
     const int vpb = (int)((nv + kUpBlocksMax - 1) / kUpBlocksMax);
     const int nblk = (int)((nv + vpb - 1) / vpb);
@@ -888,20 +981,18 @@ int up_bwd_t(const void* x, const void* dy, const float* w, void* dx, float* dw,
     k_sum_blocks<<<(n + 255) / 256, 256, 0, st>>>(part, nblk, n, dw);
     CM21_LAUNCH_CHECK("k_sum_blocks");
     const size_t vps = (size_t)8 * D * D * D;
-    k_channel_partials<<<dim3(kChanBlocks, B), 256, 0, st>>>((const bf16_t*)dy, nullptr, COUT, vps, kChanBlocks, cpart);
-    CM21_LAUNCH_CHECK("k_channel_partials");
-    k_sum_blocks<<<1, 256, 0, st>>>(cpart, B * kChanBlocks, COUT, db);
-    CM21_LAUNCH_CHECK("k_sum_blocks");
+    CM21_TRY(channel_sums(dy, nullptr, COUT, vps, B, cpart, db, nullptr, st));
     return CM21_OK;
 }
 }  // namespace
 
-extern "C" int cm21_unet_up_fwd(const void* x, const float* w, const float* bias, void* y, int B, int D, int cin,
+extern "C" int cm21_unet_up_fwd(const void* x, const void* wn, const float* bias, void* y, int B, int D, int cin,
                                 int cout, void* stream) {
-    CM21_REQUIRE(x && w && bias && y && B >= 1 && D >= 2, "cm21_unet_up_fwd: bad arguments");
+    CM21_REQUIRE(x && wn && bias && y && B >= 1 && D >= 2 && ((size_t)B * D * D * D) % 16 == 0,
+                 "cm21_unet_up_fwd: bad arguments");
     hipStream_t st = (hipStream_t)stream;
-    if (cin == 64 && cout == 32) return up_fwd_t<64, 32>(x, w, bias, y, B, D, st);
-    if (cin == 32 && cout == 16) return up_fwd_t<32, 16>(x, w, bias, y, B, D, st);
+    if (cin == 64 && cout == 32) return up_fwd_t<64, 32>(x, wn, bias, y, B, D, st);
+    if (cin == 32 && cout == 16) return up_fwd_t<32, 16>(x, wn, bias, y, B, D, st);
     return fail(CM21_ERR_INVALID, "cm21_unet_up_fwd: no kernel for cin=%d cout=%d", cin, cout);
 }
 
@@ -911,17 +1002,27 @@ extern "C" size_t cm21_unet_up_workspace_bytes(int B, int D, int cin, int cout) 
            align_up((size_t)B * kChanBlocks * cout * sizeof(float), 256);
 }
 
-extern "C" int cm21_unet_up_bwd(const void* x, const void* dy, const float* w, void* dx, float* dw, float* db, int B,
+extern "C" int cm21_unet_up_bwd(const void* x, const void* dy, const void* wk, void* dx, float* dw, float* db, int B,
                                 int D, int cin, int cout, void* work, size_t work_bytes, void* stream) {
-    CM21_REQUIRE(x && dy && w && dw && db && work && B >= 1 && D >= 2, "cm21_unet_up_bwd: bad arguments");
+    CM21_REQUIRE(x && dy && dw && db && work && B >= 1 && D >= 2 && ((size_t)B * D * D * D) % 16 == 0,
+                 "cm21_unet_up_bwd: bad arguments");
+    CM21_REQUIRE(!dx || wk, "cm21_unet_up_bwd: dx needs the packed weights wk");
     const size_t need = cm21_unet_up_workspace_bytes(B, D, cin, cout);
     if (work_bytes < need) return fail(CM21_ERR_NOMEM, "cm21_unet_up_bwd: workspace %zu < %zu", work_bytes, need);
     float* part = (float*)work;
     float* cpart = (float*)((char*)work + align_up((size_t)kUpBlocksMax * cout * 8 * cin * sizeof(float), 256));
     hipStream_t st = (hipStream_t)stream;
-    if (cin == 64 && cout == 32) return up_bwd_t<64, 32>(x, dy, w, dx, dw, db, B, D, part, cpart, st);
-    if (cin == 32 && cout == 16) return up_bwd_t<32, 16>(x, dy, w, dx, dw, db, B, D, part, cpart, st);
+    if (cin == 64 && cout == 32) return up_bwd_t<64, 32>(x, dy, wk, dx, dw, db, B, D, part, cpart, st);
+    if (cin == 32 && cout == 16) return up_bwd_t<32, 16>(x, dy, wk, dx, dw, db, B, D, part, cpart, st);
     return fail(CM21_ERR_INVALID, "cm21_unet_up_bwd: no kernel for cin=%d cout=%d", cin, cout);
+}
+
+extern "C" int cm21_unet_pack_up(const float* w, void* wn, void* wk, int cin, int cout, void* stream) {
+    CM21_REQUIRE(w && wn && wk && cin >= 1 && cout >= 1, "cm21_unet_pack_up: bad arguments");
+    const int n = cout * 8 * cin;
+    k_pack_up<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(w, (bf16_t*)wn, (bf16_t*)wk, cin, cout);
+    CM21_LAUNCH_CHECK("k_pack_up");
+    return CM21_OK;
 }
 
 extern "C" int cm21_unet_in_fwd(const float* delta, const float* cond, const float* w, const float* bias, float* beff,
@@ -967,14 +1068,7 @@ extern "C" int cm21_unet_in_wgrad(const float* delta, const float* cond, const v
     CM21_LAUNCH_CHECK("k_in_wgrad");
     k_sum_blocks<<<(16 * 27 + 255) / 256, 256, 0, st>>>(part, nblk, 16 * 27, dwd);
     CM21_LAUNCH_CHECK("k_sum_blocks");
-    const size_t vps = (size_t)D * D * D;
-    k_channel_partials<<<dim3(kChanBlocks, B), 256, 0, st>>>((const bf16_t*)g, (const bf16_t*)y, 16, vps, kChanBlocks,
-                                                              cpart);
-    CM21_LAUNCH_CHECK("k_channel_partials");
-    for (int b = 0; b < B; ++b) {
-        k_sum_blocks<<<1, 256, 0, st>>>(cpart + (size_t)b * kChanBlocks * 16, kChanBlocks, 16, gsum + b * 16);
-        CM21_LAUNCH_CHECK("k_sum_blocks");
-    }
+    CM21_TRY(channel_sums(g, y, 16, (size_t)D * D * D, B, cpart, nullptr, gsum, st));
     k_in_finish<<<(16 * 27 + 255) / 256, 256, 0, st>>>(dwd, gsum, cond, dw, db, B, 16);
     CM21_LAUNCH_CHECK("k_in_finish");
     return CM21_OK;

@@ -128,6 +128,9 @@ class UNet3D:
             if kind == "conv":
                 self.wb[name] = torch.empty((cout, 27, cin), dtype=torch.bfloat16, device=dev)
                 self.wt[name] = torch.empty((cin, 27, cout), dtype=torch.bfloat16, device=dev)
+            elif kind == "up":
+                self.wb[name] = torch.empty((8 * cout, cin), dtype=torch.bfloat16, device=dev)
+                self.wt[name] = torch.empty((cin, 8 * cout), dtype=torch.bfloat16, device=dev)
         self.step_count = 0
         self._alloc(dev)
         self.pack()
@@ -180,9 +183,13 @@ class UNet3D:
         return params, meta
 
     def pack(self):
+        """bf16 copies of the fp32 masters for the matrix-core kernels."""
         for name, kind, cin, cout in LAYERS:
             if kind == "conv":
                 _lib.call("cm21_unet_pack", _fp(self.p[name + ".w"]), _vp(self.wb[name]), _vp(self.wt[name]), cin,
+                          cout, self._st())
+            elif kind == "up":
+                _lib.call("cm21_unet_pack_up", _fp(self.p[name + ".w"]), _vp(self.wb[name]), _vp(self.wt[name]), cin,
                           cout, self._st())
 
     # -------------------------------------------------------------- layers
@@ -221,12 +228,12 @@ class UNet3D:
         _lib.call("cm21_unet_pool_fwd", _vp(a["e2"]), _vp(a["p2"]), self.B, h, 32, st)
         self._conv(a["p2"], self.wb["b1"], a["b1"], 32, 64, q, bias=p["b1.b"])
         self._conv(a["b1"], self.wb["b2"], a["b2"], 64, 64, q, bias=p["b2.b"])
-        _lib.call("cm21_unet_up_fwd", _vp(a["b2"]), _fp(p["up2.w"]), _fp(p["up2.b"]), _vp(a["u2"]), self.B, q, 64, 32,
-                  st)
+        _lib.call("cm21_unet_up_fwd", _vp(a["b2"]), _vp(self.wb["up2"]), _fp(p["up2.b"]), _vp(a["u2"]), self.B, q, 64,
+                  32, st)
         self._conv(a["e2"], self.wb["d2a"], a["d2a"], 64, 32, h, x1=a["u2"], c0=32, bias=p["d2a.b"])
         self._conv(a["d2a"], self.wb["d2b"], a["d2"], 32, 32, h, bias=p["d2b.b"])
-        _lib.call("cm21_unet_up_fwd", _vp(a["d2"]), _fp(p["up1.w"]), _fp(p["up1.b"]), _vp(a["u1"]), self.B, h, 32, 16,
-                  st)
+        _lib.call("cm21_unet_up_fwd", _vp(a["d2"]), _vp(self.wb["up1"]), _fp(p["up1.b"]), _vp(a["u1"]), self.B, h, 32,
+                  16, st)
         self._conv(a["e1"], self.wb["d1a"], a["d1a"], 32, 16, D, x1=a["u1"], c0=16, bias=p["d1a.b"])
         self._conv(a["d1a"], self.wb["d1b"], a["d1"], 16, 16, D, bias=p["d1b.b"])
         _lib.call("cm21_unet_head", _vp(a["d1"]), _fp(p["head.w"]), _fp(p["head.b"]), None, _fp(self.prob), None,
@@ -249,14 +256,14 @@ class UNet3D:
         self._conv(ga["d1"], self.wt["d1b"], ga["d1a"], 16, 16, D, mask=a["d1"], relu=False)
         self._wgrad(a["e1"], ga["d1a"], a["d1a"], "d1a", 32, 16, D, x1=a["u1"], c0=16)
         self._conv(ga["d1a"], self.wt["d1a"], ga["e1"], 16, 32, D, mask=a["d1a"], relu=False, y1=ga["u1"], o0=16)
-        _lib.call("cm21_unet_up_bwd", _vp(a["d2"]), _vp(ga["u1"]), _fp(p["up1.w"]), _vp(ga["d2"]), _fp(self.g["up1.w"]),
+        _lib.call("cm21_unet_up_bwd", _vp(a["d2"]), _vp(ga["u1"]), _vp(self.wt["up1"]), _vp(ga["d2"]), _fp(self.g["up1.w"]),
                   _fp(self.g["up1.b"]), self.B, h, 32, 16, _vp(self.work), self.work.numel(), st)
         # level 2 decoder
         self._wgrad(a["d2a"], ga["d2"], a["d2"], "d2b", 32, 32, h)
         self._conv(ga["d2"], self.wt["d2b"], ga["d2a"], 32, 32, h, mask=a["d2"], relu=False)
         self._wgrad(a["e2"], ga["d2a"], a["d2a"], "d2a", 64, 32, h, x1=a["u2"], c0=32)
         self._conv(ga["d2a"], self.wt["d2a"], ga["e2"], 32, 64, h, mask=a["d2a"], relu=False, y1=ga["u2"], o0=32)
-        _lib.call("cm21_unet_up_bwd", _vp(a["b2"]), _vp(ga["u2"]), _fp(p["up2.w"]), _vp(ga["b2"]), _fp(self.g["up2.w"]),
+        _lib.call("cm21_unet_up_bwd", _vp(a["b2"]), _vp(ga["u2"]), _vp(self.wt["up2"]), _vp(ga["b2"]), _fp(self.g["up2.w"]),
                   _fp(self.g["up2.b"]), self.B, q, 64, 32, _vp(self.work), self.work.numel(), st)
         # bottleneck
         self._wgrad(a["b1"], ga["b2"], a["b2"], "b2", 64, 64, q)
@@ -384,13 +391,27 @@ def summarise(metrics, tol_xhi=0.01, tol_p21=0.10):
 
 # ------------------------------------------------------------------ training
 
-def train(model, boxes, steps, lr=2e-3, lr_min=1e-4, seed=0, log_every=50, log=None):
+def train(model, boxes, steps, lr=2e-3, lr_min=1e-4, seed=0, log_every=50, log=None, time_budget=None):
     """AdamW with a cosine learning-rate schedule over random batches of the
-    BoxSet (order from NumPy PCG64(seed)).  Returns the per-step losses."""
+    BoxSet (order from NumPy PCG64(seed)).  With time_budget (seconds) the
+    number of steps is cut to what fits, measured on the first 10 steps.
+    Returns the logged (step, loss) pairs."""
+    import time
     rng = np.random.Generator(np.random.PCG64(int(seed)))
     losses = []
     n = len(boxes)
-    for it in range(int(steps)):
+    t0 = None
+    it = 0
+    while it < int(steps):
+        if time_budget and it == 10:
+            torch.cuda.synchronize()
+            per = (time.perf_counter() - t0) / 10
+            steps = min(int(steps), 10 + int((time_budget - 10 * per) / per))
+            if log:
+                log(f"{per * 1e3:.1f} ms/step: training for {steps} steps")
+        if it == 0:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
         sel = rng.choice(n, size=model.B, replace=False)
         model.forward(boxes.delta[sel].contiguous(), boxes.cond[sel])
         loss = model.backward(boxes.target[sel].contiguous())
@@ -401,6 +422,7 @@ def train(model, boxes, steps, lr=2e-3, lr_min=1e-4, seed=0, log_every=50, log=N
             losses.append((it, lv))
             if log:
                 log(f"step {it} loss {lv:.5f} lr {cur:.2e}")
+        it += 1
     return losses
 
 
@@ -460,3 +482,63 @@ class SurrogatePipeline:
                         self.disabled = True
                     return self._plain(delta, zeta, smin2), "plain"
         return prob, "surrogate"
+
+
+def _seed_range(text):
+    a, b = (int(v) for v in text.split(":"))
+    return range(a, b)
+
+
+def main(argv=None):
+    """Train the surrogate on plain-method boxes and evaluate it on held-out
+    seeds: python -m 21cmpy_amd.surrogate --steps 4000 --out weights.npz"""
+    import argparse
+    import time
+    ap = argparse.ArgumentParser(description="train / evaluate the U-Net surrogate")
+    ap.add_argument("--D", type=int, default=128)
+    ap.add_argument("--box-len", type=float, default=200.0)
+    ap.add_argument("--deltac", type=float, default=4.0)
+    ap.add_argument("--train-seeds", default="1000:3000")
+    ap.add_argument("--eval-seeds", default="3000:3200")
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--time-budget", type=float, default=0.0, help="seconds of training at most")
+    ap.add_argument("--init", default=None, help="start from these weights")
+    ap.add_argument("--out", default=None, help="weights file to write")
+    ap.add_argument("--eval-only", action="store_true")
+    args = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    t = time.perf_counter()
+    params = UNet3D.load_file(args.init)[0] if args.init else None
+    model = UNet3D(D=args.D, batch=args.batch, params=params)
+    report = {"D": args.D, "box_len": args.box_len, "deltac": args.deltac, "batch": args.batch}
+    if not args.eval_only:
+        train_set = BoxSet(surrogate_specs(_seed_range(args.train_seeds), seed=7, D=args.D, box_len=args.box_len,
+                                           deltac=args.deltac))
+        xh = train_set.x_HI()
+        log(f"training boxes: {len(train_set)} in {time.perf_counter() - t:.1f} s; x_HI quartiles "
+            f"{np.round(np.quantile(xh, [0, 0.25, 0.5, 0.75, 1]), 3).tolist()}")
+        t = time.perf_counter()
+        losses = train(model, train_set, args.steps, lr=args.lr, log_every=50, log=log,
+                       time_budget=args.time_budget or None)
+        torch.cuda.synchronize()
+        report["train"] = {"boxes": len(train_set), "steps": losses[-1][0] + 1, "seconds": time.perf_counter() - t,
+                           "loss_first": losses[0][1], "loss_last": losses[-1][1]}
+        del train_set
+        torch.cuda.empty_cache()
+        if args.out:
+            model.save(args.out, **{k: v for k, v in report.items() if k != "train"})
+            log(f"weights written to {args.out}")
+    eval_set = BoxSet(surrogate_specs(_seed_range(args.eval_seeds), seed=8, D=args.D, box_len=args.box_len,
+                                      deltac=args.deltac))
+    report["eval"] = summarise(evaluate(model, eval_set))
+    log(json.dumps(report))
+    return report
+
+
+if __name__ == "__main__":
+    main()

@@ -159,6 +159,120 @@ def batched_bench(args, torch):
     }
 
 
+def unet_flops(D):
+    """FLOPs of one surrogate forward pass at side D: 2 x MACs of the 3^3
+    convolutions (matrix cores), the transposed convolutions, the input
+    layer and the head."""
+    h, q = D // 2, D // 4
+    conv = [(D, 16, 16), (h, 16, 32), (h, 32, 32), (q, 32, 64), (q, 64, 64), (h, 64, 32), (h, 32, 32),
+            (D, 32, 16), (D, 16, 16)]
+    mfma = sum(2 * 27 * ci * co * s ** 3 for s, ci, co in conv)
+    other = 2 * 27 * 16 * D ** 3 + 2 * 64 * 32 * h ** 3 + 2 * 32 * 16 * D ** 3 + 2 * 16 * D ** 3
+    return mfma, other
+
+
+def plain_rate(torch, delta, L, deltac, smin, zeta, steps, linear=True):
+    """Boxes/s of the plain single-box loop (fused path, graph replay) on one
+    resident box, radii and denominators computed beforehand."""
+    lib = importlib.import_module("21cmpy_amd._lib")
+    fb = importlib.import_module("21cmpy_amd.Programs.find_HII_bubbles")
+    sig = importlib.import_module("21cmpy_amd.cosmo_files.sigmas")
+    D = delta.shape[0]
+    radii, denoms, fc_mean = fb.bubble_setup(D, L, None, deltac, smin, sig.PowerLawSpectrum(D, L).sig0_discrete)
+    prm = lib.BubbleParams(box_len=L, deltac=float(np.float32(deltac)), zeta=zeta,
+                           growth_inv=float(np.float32(deltac / 1.686)), fc_mean=float(fc_mean),
+                           linear_input=1 if linear else 0, filter_type=1, ref_fallthrough=1, update=0,
+                           sphere_periodic=0, sphere_r_scale=1.0, fused=1, graph=1)
+    plan = lib.plan(D, 1)
+    nbytes = lib.query("cm21_find_bubbles_workspace_bytes", D, len(radii), 0)
+    work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    ion = torch.empty_like(delta)
+    r_arr = np.ascontiguousarray(radii, np.float32)
+    d_arr = np.ascontiguousarray(denoms, np.float32)
+    stream = torch.cuda.Stream()
+
+    def step():
+        lib.call("cm21_find_bubbles", plan.handle, lib.ptr(delta), lib.ptr(ion), r_arr.ctypes.data_as(lib.c_float_p),
+                 d_arr.ctypes.data_as(lib.c_float_p), len(radii), prm, lib.ptr(work), nbytes,
+                 lib.stream_handle(stream))
+
+    torch.cuda.synchronize()
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    return steps / (time.perf_counter() - t0)
+
+
+def surrogate_bench(args, torch, plain_boxes_s=None):
+    """U-Net surrogate at HII_DIM=128 (BOX_LEN=200 Mpc): inference boxes/s at
+    batch 8, a training step's time, and accuracy against the plain method
+    on held-out seeds (3000..3015; trained on 1000..2999)."""
+    sur = importlib.import_module("21cmpy_amd.surrogate")
+    D, L, B = 128, 200.0, 8
+    trained = os.path.exists(sur.WEIGHTS)
+    params, meta = sur.UNet3D.load_file(sur.WEIGHTS) if trained else (None, {})
+    model = sur.UNet3D(D=D, batch=B, params=params)
+    specs = sur.surrogate_specs(range(3000, 3016), seed=8, D=D, box_len=L)
+    boxes = sur.BoxSet(specs)
+    delta = boxes.delta[:B].contiguous()
+    cond = boxes.cond[:B]
+    for _ in range(2):
+        model.forward(delta, cond)
+    steps = max(args.steps, 1)
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(steps):
+        model.forward(delta, cond)
+    end.record()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    fwd_ms = start.elapsed_time(end) / steps
+    value = B * steps / elapsed
+    # a training step (forward, backward, AdamW) on a copy of the model
+    tm = sur.UNet3D(D=D, batch=B, params=params)
+    tgt = boxes.target[:B].contiguous()
+    tm.forward(delta, cond)
+    tm.backward(tgt)
+    tm.step(1e-4)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(3):
+        tm.forward(delta, cond)
+        tm.backward(tgt)
+        tm.step(1e-4)
+    torch.cuda.synchronize()
+    train_ms = (time.perf_counter() - t1) * 1e3 / 3
+    del tm
+    summary = sur.summarise(sur.evaluate(model, boxes)) if trained else None
+    # plain method on the same box size, one box per call through the C ABI
+    # (graph replay), host set-up outside the timed region
+    s0 = specs[0]
+    plain128 = plain_rate(torch, boxes.delta[0].contiguous(), L, s0["deltac"], s0["sigma_min2"], s0["zeta"],
+                          steps)
+    mf, other = unet_flops(D)
+    achieved = B * (mf + other) / (fwd_ms * 1e-3) / 1e12
+    out = {
+        "value": value, "unit": "boxes/s", "HII_DIM": D, "BOX_LEN": L, "batch": B, "steps": steps,
+        "ms_per_forward_batch": fwd_ms, "ms_per_train_step_batch": train_ms,
+        "trained": trained, "weights_meta": meta,
+        "accuracy_vs_plain_heldout16": summary,
+        "plain_same_HII_DIM_boxes_s": plain128,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": achieved / 2500.0, "flops_per_box": mf + other, "mfma_flops_per_box": mf},
+        "workload": "3-level 16/32/64 3D U-Net, bf16 MFMA convs + fp32 accumulation, on linear GRF boxes "
+                    "(zeta in [10,60], sigma_min^2 in [2,6], barrier 4.0)",
+    }
+    if plain_boxes_s:
+        out["vs_plain_256"] = value / plain_boxes_s
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,7 +285,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-box batched measurement")
     ap.add_argument("--no-tune", action="store_true", help="batched: use the stored tile choice, no autotuning")
     ap.add_argument("--save-tiles", default=None, help="batched: merge the autotuned tiles into this JSON file")
-    ap.add_argument("--mode", choices=["bench", "loop", "copy", "batched"], default="bench",
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the U-Net surrogate measurement")
+    ap.add_argument("--mode", choices=["bench", "loop", "copy", "batched", "surrogate"], default="bench",
                     help="loop: only the radius loop on a GRF box (for counter runs); copy: a 512 MiB device "
                          "copy (to calibrate FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -198,6 +313,9 @@ def main():
         return loop_only(args, lib, torch)
     if args.mode == "batched":
         print(json.dumps({"mode": "batched", "batched": batched_bench(args, torch)}))
+        return
+    if args.mode == "surrogate":
+        print(json.dumps({"mode": "surrogate", "surrogate": surrogate_bench(args, torch)}))
         return
     init = importlib.import_module("21cmpy_amd.Programs.initialize")
     pert = importlib.import_module("21cmpy_amd.Programs.perturb_field")
@@ -339,6 +457,9 @@ def main():
         torch.cuda.empty_cache()
         out["batched"] = batched_bench(args, torch)
         out["batched"]["vs_plain"] = out["batched"]["value"] / boxes_s
+    if not args.no_surrogate:
+        torch.cuda.empty_cache()
+        out["surrogate"] = surrogate_bench(args, torch, boxes_s)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(upd.cpu().numpy(), L, radii, denoms, zeta, deltac, fc_mean,
                                            args.cpu_budget)

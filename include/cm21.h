@@ -314,13 +314,16 @@ int cm21_unet_conv3_wgrad(const cm21_unet_wgrad* w, void* stream);
 int cm21_unet_pool_fwd(const void* x, void* y, int B, int D, int C, void* stream);
 int cm21_unet_pool_bwd(const void* dy, void* dx, int B, int D, int C, int accumulate, void* stream);
 
-/* 2x2x2 stride-2 transposed conv, x (B,D,D,D,cin) -> y (B,2D,2D,2D,cout),
- * w fp32 [cout][8][cin], sub-voxel s = 4a + 2b + c; (cin, cout) in
+/* 2x2x2 stride-2 transposed conv on the matrix cores, x (B,D,D,D,cin) ->
+ * y (B,2D,2D,2D,cout); master weights fp32 w [cout][8][cin] with sub-voxel
+ * s = 4a + 2b + c, packed by cm21_unet_pack_up into wn (bf16 [8 cout][cin],
+ * forward) and wk (bf16 [cin][8 cout], input gradient); (cin, cout) in
  * {(64,32), (32,16)}.  Backward: dx (or NULL), dw and db overwritten. */
-int cm21_unet_up_fwd(const void* x, const float* w, const float* bias, void* y, int B, int D, int cin, int cout,
+int cm21_unet_pack_up(const float* w, void* wn, void* wk, int cin, int cout, void* stream);
+int cm21_unet_up_fwd(const void* x, const void* wn, const float* bias, void* y, int B, int D, int cin, int cout,
                      void* stream);
 size_t cm21_unet_up_workspace_bytes(int B, int D, int cin, int cout);
-int cm21_unet_up_bwd(const void* x, const void* dy, const float* w, void* dx, float* dw, float* db, int B, int D,
+int cm21_unet_up_bwd(const void* x, const void* dy, const void* wk, void* dx, float* dw, float* db, int B, int D,
                      int cin, int cout, void* work, size_t work_bytes, void* stream);
 
 /* Input layer: 16 channels from fp32 density delta (B,D,D,D) scaled by

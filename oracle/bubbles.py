@@ -205,14 +205,23 @@ def find_bubbles_f32(delta, box_len, radii, denoms, zeta, deltac, LE=True,
 
 
 def find_bubbles_f64(delta, box_len, radii, denoms, zeta, deltac, LE=True,
-                     growth_inv=1.0, fc_mean=None, fil=1, ref_fallthrough=True):
+                     growth_inv=1.0, fc_mean=None, fil=1, ref_fallthrough=True, fft_workers=None):
     """FP64 version of the centre-pixel loop; also returns, per cell, the
-    minimum over radii of |zeta*f_coll - 1| (the threshold margin)."""
+    minimum over radii of |zeta*f_coll - 1| (the threshold margin).
+    fft_workers: run the transforms with scipy.fft on that many threads
+    (same results up to FP64 rounding; for large boxes)."""
+    if fft_workers:
+        import scipy.fft as sfft
+        rfftn = lambda a: sfft.rfftn(a, workers=fft_workers)  # noqa: E731
+        irfftn = lambda a, s: sfft.irfftn(a, s=s, axes=(0, 1, 2), workers=fft_workers)  # noqa: E731
+    else:
+        rfftn = np.fft.rfftn
+        irfftn = lambda a, s: np.fft.irfftn(a, s=s, axes=(0, 1, 2))  # noqa: E731
     d = np.asarray(delta, dtype=np.float64)
     D = d.shape[0]
     if not LE:
         d = d * float(growth_inv)
-    spec = np.fft.rfftn(d)
+    spec = rfftn(d)
     dk = 2.0 * np.pi / float(box_len)
     n0 = mode_index(D).astype(np.float64)
     n2 = np.arange(D // 2 + 1, dtype=np.float64)
@@ -222,7 +231,7 @@ def find_bubbles_f64(delta, box_len, radii, denoms, zeta, deltac, LE=True,
     nR = len(radii)
     for r, (R, denom) in enumerate(zip(radii, denoms)):
         w = window_f64(kmag, float(R), fil, ref_fallthrough)
-        sm = np.fft.irfftn(spec * w, s=(D, D, D), axes=(0, 1, 2))
+        sm = irfftn(spec * w, (D, D, D))
         fc = _erfc((float(deltac) - sm) / float(denom))
         if not LE:
             fc = fc * (float(fc_mean) / fc.mean())

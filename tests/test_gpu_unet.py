@@ -147,19 +147,24 @@ def test_pool_and_up(gpu):
         xi = _bf((B, Din, Din, Din, cin), 0.8, 24)
         w = _f((cout, 8, cin), np.sqrt(1.0 / cin), 25)
         bia = _f((cout,), 0.1, 26)
+        wn = torch.empty((8 * cout, cin), dtype=torch.bfloat16, device="cuda")
+        wk = torch.empty((cin, 8 * cout), dtype=torch.bfloat16, device="cuda")
+        lib.call("cm21_unet_pack_up", _fp(w), _vp(wn), _vp(wk), cin, cout, lib.stream_handle())
         yo = torch.empty((B, 2 * Din, 2 * Din, 2 * Din, cout), dtype=torch.bfloat16, device="cuda")
-        lib.call("cm21_unet_up_fwd", _vp(xi), _fp(w), _fp(bia), _vp(yo), B, Din, cin, cout, lib.stream_handle())
+        lib.call("cm21_unet_up_fwd", _vp(xi), _vp(wn), _fp(bia), _vp(yo), B, Din, cin, cout, lib.stream_handle())
         g = _bf((B, 2 * Din, 2 * Din, 2 * Din, cout), 0.5, 27)
         gx = torch.empty_like(xi)
         dw = torch.empty_like(w)
         db = torch.empty_like(bia)
         work = torch.empty(lib.query("cm21_unet_up_workspace_bytes", B, Din, cin, cout), dtype=torch.uint8,
                            device="cuda")
-        lib.call("cm21_unet_up_bwd", _vp(xi), _vp(g), _fp(w), _vp(gx), _fp(dw), _fp(db), B, Din, cin, cout,
+        lib.call("cm21_unet_up_bwd", _vp(xi), _vp(g), _vp(wk), _vp(gx), _fp(dw), _fp(db), B, Din, cin, cout,
                  _vp(work), work.numel(), lib.stream_handle())
         torch.cuda.synchronize()
+        # the matrix cores use the bf16 copy of the weights
+        assert torch.equal(wn.cpu().float(), w.cpu().permute(1, 0, 2).reshape(8 * cout, cin).to(torch.bfloat16).float())
         xr = _d(xi).requires_grad_(True)
-        wr = _d(w).requires_grad_(True)
+        wr = _d(w.to(torch.bfloat16)).requires_grad_(True)
         br = _d(bia).requires_grad_(True)
         out = unet.up(xr, wr, br)
         _close_bf16(yo, out.detach(), f"up fwd {cin}->{cout}")
@@ -299,3 +304,47 @@ def test_training_reduces_loss_on_plain_targets(gpu):
     last = np.mean([l for _, l in losses[-5:]])
     print(f"loss {first:.4f} -> {last:.4f}", flush=True)
     assert last < 0.8 * first
+
+
+def _heldout(n, D=128, first=3000):
+    sur = pkg("surrogate")
+    return sur.BoxSet(sur.surrogate_specs(range(first, first + n), seed=8, D=D, box_len=200.0))
+
+
+def test_trained_network_gains_over_untrained(gpu):
+    """The committed weights (trained on seeds 1000..2999) against an
+    untrained network of the same shape on held-out seeds: the trained one
+    reproduces the plain method's neutral fraction and ionized cells, the
+    untrained one does not."""
+    sur = pkg("surrogate")
+    params, meta = sur.UNet3D.load_file(sur.WEIGHTS)
+    boxes = _heldout(8)
+    trained = sur.summarise(sur.evaluate(sur.UNet3D(D=128, batch=4, params=params), boxes))
+    untrained = sur.summarise(sur.evaluate(sur.UNet3D(D=128, batch=4, seed=123), boxes))
+    print("trained", trained, "\nuntrained", untrained, flush=True)
+    assert trained["dx_HI_mean"] < 0.02 and trained["agreement_mean"] > 0.95
+    assert trained["dx_HI_mean"] < 0.25 * untrained["dx_HI_mean"]
+    assert trained["agreement_mean"] > untrained["agreement_mean"] + 0.1
+
+
+def test_pipeline_validates_and_falls_back(gpu):
+    """Served with the trained network, sampled boxes are validated against
+    the plain method; an untrained network fails validation, its requests
+    are answered by the plain method, and after max_failures it is bypassed."""
+    sur = pkg("surrogate")
+    params, _ = sur.UNet3D.load_file(sur.WEIGHTS)
+    boxes = _heldout(4)
+    zeta = [s["zeta"] for s in boxes.specs]
+    smin = [s["sigma_min2"] for s in boxes.specs]
+    good = sur.SurrogatePipeline(sur.UNet3D(D=128, batch=4, params=params), validate_every=2, tol_xhi=0.05,
+                                 tol_p21=10.0)
+    fields, src = good(boxes.delta, zeta, smin)
+    assert good.validated == 2
+    assert src == "surrogate" and good.fallbacks == 0
+    bad = sur.SurrogatePipeline(sur.UNet3D(D=128, batch=4, seed=5), validate_every=1, tol_xhi=0.01, max_failures=2)
+    for rep in range(3):
+        fields, src = bad(boxes.delta, zeta, smin)
+        assert src == "plain"
+        for j in range(4):
+            assert torch.equal(fields[j], boxes.target[j]), (rep, j)
+    assert bad.failures == 2 and bad.disabled and bad.fallbacks == 3
