@@ -120,11 +120,23 @@ __global__ void k_fill_u64(unsigned long long* __restrict__ p, size_t n) {
 }
 
 
-__global__ void k_sum_partials_lines(const double* __restrict__ partial, int n, double* __restrict__ out) {
+// One block of 1024 threads: strided per-thread sums, then a fixed-order tree
+// over the 16 wavefronts (deterministic).
+constexpr int kSumThreads = 1024;
+
+__global__ __launch_bounds__(kSumThreads) void k_sum_partials_lines(const double* __restrict__ partial, int n,
+                                                                    double* __restrict__ out) {
+    __shared__ double part[kSumThreads / 64];
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
-    acc = block_sum(acc);
-    if (threadIdx.x == 0) *out = acc;
+    for (int i = threadIdx.x; i < n; i += kSumThreads) acc += partial[i];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < kSumThreads / 64; ++w) r += part[w];
+        *out = r;
+    }
 }
 
 // ------------------------------------------------------------------ host side
@@ -266,7 +278,7 @@ LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double*
 }
 
 int fused_sum(const double* partial, int n, double* out, hipStream_t st) {
-    k_sum_partials_lines<<<1, kThreads, 0, st>>>(partial, n, out);
+    k_sum_partials_lines<<<1, kSumThreads, 0, st>>>(partial, n, out);
     CM21_LAUNCH_CHECK("k_sum_partials_lines");
     return CM21_OK;
 }

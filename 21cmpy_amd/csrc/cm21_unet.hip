@@ -180,6 +180,164 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
     }
 }
 
+// Sliding-window form for large boxes: a block owns 16 x TY output voxels of
+// NZ consecutive z-planes and walks z with a ring of four input planes (each
+// (TY+2) x 18 voxels with halo) in LDS: one new plane is fetched per output
+// plane, into registers while the previous plane computes.  Layers with at
+// most 14 weight fragments (K slices x N tiles) keep every B fragment in
+// registers for the whole walk; the others stream them per K slice,
+// double-buffered.
+template <int CIN, int COUT, int TY, int NZ>
+__global__ __launch_bounds__(256) void k_conv3_slide(ConvArgs a) {
+    constexpr int HY = TY + 2, HX = 18, CP = CIN >= 32 ? CIN + 8 : CIN;
+    constexpr int PLANE = HY * HX * CP;               // bf16 per ring slot
+    constexpr int MW = TY / 4, NT = COUT / 16;
+    constexpr int K = 27 * CIN, KC = (K + 31) / 32;
+    constexpr bool HOLD = KC * NT <= 14;
+    constexpr int CHUNKS = HY * HX * (CIN / 8);       // 16-byte chunks per plane
+    constexpr int NPF = (CHUNKS + 255) / 256;
+    static_assert(TY % 4 == 0, "four waves share the lines");
+    static_assert(CIN % 8 == 0 && COUT % 16 == 0, "channel counts");
+    __shared__ __attribute__((aligned(16))) bf16_t ring[4 * PLANE];
+    const int D = a.D;
+    const int nyt = D / TY;
+    const int x0 = blockIdx.x * 16;
+    const int y0 = (blockIdx.y % nyt) * TY, z0 = (blockIdx.y / nyt) * NZ;
+    const int b = blockIdx.z;
+    const int c1 = CIN - a.c0;
+    const int t = threadIdx.x;
+    const u32x4 zero = {0u, 0u, 0u, 0u};
+
+    // global -> registers for halo plane hz (global z = z0 - 1 + hz)
+    auto fetch = [&](int hz, u32x4 (&pf)[NPF]) {
+        const int gz = wrap(z0 - 1 + hz, D);
+#pragma unroll
+        for (int q = 0; q < NPF; ++q) {
+            const int i = t + q * 256;
+            u32x4 v = zero;
+            if (i < CHUNKS) {
+                const int ch = (i % (CIN / 8)) * 8;
+                const int vox = i / (CIN / 8);
+                const int hx = vox % HX, hy = vox / HX;
+                const size_t g = (((size_t)b * D + gz) * D + wrap(y0 + hy - 1, D)) * D + wrap(x0 + hx - 1, D);
+                if (ch < a.c0) {
+                    v = *(const u32x4*)(a.x0 + g * a.c0 + ch);
+                    if (a.mask) v = mask8(v, *(const u32x4*)(a.mask + g * a.c0 + ch));
+                } else {
+                    v = *(const u32x4*)(a.x1 + g * c1 + (ch - a.c0));
+                }
+            }
+            pf[q] = v;
+        }
+    };
+    auto store = [&](int hz, const u32x4 (&pf)[NPF]) {
+        bf16_t* slot = ring + (hz & 3) * PLANE;
+#pragma unroll
+        for (int q = 0; q < NPF; ++q) {
+            const int i = t + q * 256;
+            if (i < CHUNKS) {
+                const int ch = (i % (CIN / 8)) * 8;
+                const int vox = i / (CIN / 8);
+                *(u32x4*)(slot + vox * CP + ch) = pf[q];
+            }
+        }
+    };
+    {
+        u32x4 pf[NPF];
+        for (int hz = 0; hz < 3; ++hz) {
+            fetch(hz, pf);
+            store(hz, pf);
+        }
+    }
+    const int lane = t & 63, wave = t >> 6;
+    const int row = lane & 15, grp = lane >> 4;
+    auto b_frag = [&](int kc, int n) -> bf16x8 {
+        const int kraw = kc * 32 + grp * 8;
+        return __builtin_bit_cast(bf16x8, kraw < K ? *(const u32x4*)(a.w + (size_t)(n * 16 + row) * K + kraw) : zero);
+    };
+    bf16x8 bhold[HOLD ? KC : 1][NT];
+    if constexpr (HOLD) {
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+            for (int n = 0; n < NT; ++n) bhold[kc][n] = b_frag(kc, n);
+    }
+    __syncthreads();
+    const int o1 = COUT - a.o0;
+    for (int oz = 0; oz < NZ; ++oz) {
+        u32x4 pf[NPF];
+        const bool more = oz + 3 <= NZ + 1;
+        if (more) fetch(oz + 3, pf);
+        floatx4 acc[MW][NT];
+#pragma unroll
+        for (int m = 0; m < MW; ++m)
+#pragma unroll
+            for (int n = 0; n < NT; ++n) acc[m][n] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        // one K slice: A fragments of this wave's lines from the ring, MFMAs
+        auto slice = [&](int kc, const bf16x8 (&bw)[NT]) {
+            const int kraw = kc * 32 + grp * 8;
+            const bool valid = kraw < K;
+            const int k0 = valid ? kraw : 0;
+            const int tap = k0 / CIN, ci = k0 % CIN;
+            const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+            const bf16_t* slot = ring + ((oz + dz) & 3) * PLANE;
+            bf16x8 av[MW];
+#pragma unroll
+            for (int m = 0; m < MW; ++m) {
+                const int ly = wave * MW + m;
+                av[m] = __builtin_bit_cast(bf16x8, valid ? *(const u32x4*)(slot + ((ly + dy) * HX + row + dx) * CP + ci)
+                                                         : zero);
+            }
+#pragma unroll
+            for (int m = 0; m < MW; ++m)
+#pragma unroll
+                for (int n = 0; n < NT; ++n)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bw[n], acc[m][n], 0, 0, 0);
+        };
+        if constexpr (HOLD) {
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) slice(kc, bhold[kc]);
+        } else {
+            bf16x8 bcur[NT], bnext[NT];
+#pragma unroll
+            for (int n = 0; n < NT; ++n) bcur[n] = b_frag(0, n);
+#pragma unroll 2
+            for (int kc = 0; kc < KC; ++kc) {
+                if (kc + 1 < KC) {
+#pragma unroll
+                    for (int n = 0; n < NT; ++n) bnext[n] = b_frag(kc + 1, n);
+                }
+                slice(kc, bcur);
+#pragma unroll
+                for (int n = 0; n < NT; ++n) bcur[n] = bnext[n];
+            }
+        }
+        const int z = z0 + oz;
+#pragma unroll
+        for (int m = 0; m < MW; ++m) {
+            const int y = y0 + wave * MW + m;
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                const int co = n * 16 + row;
+                const float bv = a.bias ? a.bias[b * a.bias_stride + co] : 0.0f;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int x = x0 + grp * 4 + i;
+                    float v = acc[m][n][i] + bv;
+                    if (a.relu) v = fmaxf(v, 0.0f);
+                    const size_t vox = (((size_t)b * D + z) * D + y) * D + x;
+                    if (co < a.o0)
+                        a.y0[vox * a.o0 + co] = f2bf(v);
+                    else
+                        a.y1[vox * o1 + (co - a.o0)] = f2bf(v);
+                }
+            }
+        }
+        if (more) store(oz + 3, pf);
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------ 3^3 conv weight gradient
 
 struct WgradArgs {
@@ -799,22 +957,29 @@ void launch_conv(const ConvArgs& a, hipStream_t st) {
     const dim3 grid(a.D / 16, (a.D / TZ) * (a.D / TY), a.B);
     k_conv3_mfma<CIN, COUT, TZ, TY><<<grid, 256, 0, st>>>(a);
 }
+template <int CIN, int COUT, int TY, int NZ>
+void launch_slide(const ConvArgs& a, hipStream_t st) {
+    const dim3 grid(a.D / 16, (a.D / TY) * (a.D / NZ), a.B);
+    k_conv3_slide<CIN, COUT, TY, NZ><<<grid, 256, 0, st>>>(a);
+}
 
+// Tiled form: 4 x 4 x 16 (cin <= 32) or 4 x 2 x 16 (cin = 64) voxels per
+// block for D >= 64, 2 x 2 x 16 below (enough blocks to fill the chip).
+// Sliding form: 8 (cin <= 32) or 4 lines of 16 voxels over 16 planes
+// (D >= 128) or 8 planes (D = 64).
 struct ConvShape {
     int cin, cout;
-    ConvFn big, small;  // D >= 64 / D < 64
-    int tz_big, ty_big, tz_small, ty_small;
+    ConvFn tile_big, tile_small, slide128, slide64;
 };
 
+#define CM21_CONV(CI, CO)                                                                             \
+    {CI, CO, launch_conv<CI, CO, 4, (CI >= 64 ? 2 : 4)>, launch_conv<CI, CO, 2, 2>,                    \
+     launch_slide<CI, CO, (CI >= 64 ? 4 : 8), 16>, launch_slide<CI, CO, (CI >= 64 ? 4 : 8), 8>}
 const ConvShape kConvShapes[] = {
-    {16, 16, launch_conv<16, 16, 4, 4>, launch_conv<16, 16, 2, 2>, 4, 4, 2, 2},
-    {16, 32, launch_conv<16, 32, 4, 4>, launch_conv<16, 32, 2, 2>, 4, 4, 2, 2},
-    {32, 16, launch_conv<32, 16, 4, 4>, launch_conv<32, 16, 2, 2>, 4, 4, 2, 2},
-    {32, 32, launch_conv<32, 32, 4, 4>, launch_conv<32, 32, 2, 2>, 4, 4, 2, 2},
-    {32, 64, launch_conv<32, 64, 4, 2>, launch_conv<32, 64, 2, 2>, 4, 2, 2, 2},
-    {64, 32, launch_conv<64, 32, 4, 2>, launch_conv<64, 32, 2, 2>, 4, 2, 2, 2},
-    {64, 64, launch_conv<64, 64, 4, 2>, launch_conv<64, 64, 2, 2>, 4, 2, 2, 2},
+    CM21_CONV(16, 16), CM21_CONV(16, 32), CM21_CONV(32, 16), CM21_CONV(32, 32),
+    CM21_CONV(32, 64), CM21_CONV(64, 32), CM21_CONV(64, 64),
 };
+#undef CM21_CONV
 
 const ConvShape* conv_shape(int cin, int cout) {
     for (const auto& s : kConvShapes)
@@ -890,7 +1055,15 @@ extern "C" int cm21_unet_conv3_fwd(const cm21_unet_conv* c, void* stream) {
     a.y0 = (bf16_t*)c->y0;
     a.y1 = (bf16_t*)c->y1;
     hipStream_t st = (hipStream_t)stream;
-    (c->D >= 64 ? s->big : s->small)(a, st);
+    CM21_REQUIRE(c->algo >= 0 && c->algo <= 2, "cm21_unet_conv3_fwd: algo=%d", c->algo);
+    CM21_REQUIRE(c->algo != 2 || c->D >= 64, "cm21_unet_conv3_fwd: the sliding form needs D >= 64");
+    // automatic choice from per-layer timings on the MI355X: the sliding
+    // window wins for 16 -> 16 channels at D >= 128, one tile per block
+    // elsewhere
+    const int algo = c->algo ? c->algo : ((c->cin == 16 && c->cout == 16 && c->D >= 128) ? 2 : 1);
+    ConvFn fn = c->D < 64 ? s->tile_small
+                          : (algo == 1 ? s->tile_big : (c->D >= 128 ? s->slide128 : s->slide64));
+    fn(a, st);
     CM21_LAUNCH_CHECK("k_conv3_mfma");
     return CM21_OK;
 }

@@ -391,11 +391,29 @@ def summarise(metrics, tol_xhi=0.01, tol_p21=0.10):
 
 # ------------------------------------------------------------------ training
 
-def train(model, boxes, steps, lr=2e-3, lr_min=1e-4, seed=0, log_every=50, log=None, time_budget=None):
+def _symmetry(delta, target, rng):
+    """One random symmetry of the periodic cube applied to a batch of
+    (B, D, D, D) boxes and their targets: an axis permutation, reflections
+    and a periodic shift (the excursion-set field is equivariant under all
+    of them: the filters depend on |k| only)."""
+    perm = [int(a) + 1 for a in rng.permutation(3)]
+    delta = delta.permute(0, *perm)
+    target = target.permute(0, *perm)
+    flips = [a + 1 for a in range(3) if rng.random() < 0.5]
+    if flips:
+        delta, target = delta.flip(flips), target.flip(flips)
+    D = delta.shape[1]
+    shift = tuple(int(v) for v in rng.integers(0, D, size=3))
+    return delta.roll(shift, dims=(1, 2, 3)), target.roll(shift, dims=(1, 2, 3))
+
+
+def train(model, boxes, steps, lr=2e-3, lr_min=1e-4, seed=0, log_every=50, log=None, time_budget=None,
+          augment=False):
     """AdamW with a cosine learning-rate schedule over random batches of the
-    BoxSet (order from NumPy PCG64(seed)).  With time_budget (seconds) the
-    number of steps is cut to what fits, measured on the first 10 steps.
-    Returns the logged (step, loss) pairs."""
+    BoxSet (order from NumPy PCG64(seed)), optionally with a random symmetry
+    of the periodic cube per batch.  With time_budget (seconds) the number of
+    steps is cut to what fits, measured on the first 10 steps.  Returns the
+    logged (step, loss) pairs."""
     import time
     rng = np.random.Generator(np.random.PCG64(int(seed)))
     losses = []
@@ -413,8 +431,11 @@ def train(model, boxes, steps, lr=2e-3, lr_min=1e-4, seed=0, log_every=50, log=N
             torch.cuda.synchronize()
             t0 = time.perf_counter()
         sel = rng.choice(n, size=model.B, replace=False)
-        model.forward(boxes.delta[sel].contiguous(), boxes.cond[sel])
-        loss = model.backward(boxes.target[sel].contiguous())
+        delta, target = boxes.delta[sel], boxes.target[sel]
+        if augment:
+            delta, target = _symmetry(delta, target, rng)
+        model.forward(delta.contiguous(), boxes.cond[sel])
+        loss = model.backward(target.contiguous())
         cur = lr_min + 0.5 * (lr - lr_min) * (1.0 + math.cos(math.pi * it / max(int(steps) - 1, 1)))
         model.step(cur)
         if log_every and (it % log_every == 0 or it == steps - 1):
@@ -424,6 +445,29 @@ def train(model, boxes, steps, lr=2e-3, lr_min=1e-4, seed=0, log_every=50, log=N
                 log(f"step {it} loss {lv:.5f} lr {cur:.2e}")
         it += 1
     return losses
+
+
+def calibrate_logit_scale(model, boxes, scales=(1.0, 1.25, 1.5, 1.75, 2.0, 2.5, 3.0), idx=None, log=None):
+    """Temperature scaling of the output: multiply the head's weights and bias
+    by the scale a (p = sigmoid(a * logit)) that minimises the median P21(k)
+    error on the given (training) boxes, among those that keep the mean
+    |dx_HI| within 10% of the unscaled one.  Applies the chosen scale to the
+    model and returns (scale, {scale: summary})."""
+    w0 = model.p["head.w"].clone()
+    b0 = model.p["head.b"].clone()
+    table = {}
+    for a in scales:
+        model.p["head.w"].copy_(w0 * a)
+        model.p["head.b"].copy_(b0 * a)
+        table[a] = summarise(evaluate(model, boxes, idx))
+        if log:
+            log(f"logit scale {a}: {table[a]}")
+    base = table[scales[0]]["dx_HI_mean"]
+    ok = [a for a in scales if table[a]["dx_HI_mean"] <= 1.1 * base + 1e-4]
+    best = min(ok, key=lambda a: table[a]["p21_err_median"])
+    model.p["head.w"].copy_(w0 * best)
+    model.p["head.b"].copy_(b0 * best)
+    return best, table
 
 
 # ------------------------------------------------------------------ serving with fallback
@@ -503,10 +547,14 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr-min", type=float, default=1e-4)
+    ap.add_argument("--augment", action="store_true", help="random cube symmetry per batch")
     ap.add_argument("--time-budget", type=float, default=0.0, help="seconds of training at most")
     ap.add_argument("--init", default=None, help="start from these weights")
     ap.add_argument("--out", default=None, help="weights file to write")
     ap.add_argument("--eval-only", action="store_true")
+    ap.add_argument("--calibrate", type=int, default=0,
+                    help="fit the output temperature on this many training boxes after training")
     args = ap.parse_args(argv)
 
     def log(msg):
@@ -523,15 +571,31 @@ def main(argv=None):
         log(f"training boxes: {len(train_set)} in {time.perf_counter() - t:.1f} s; x_HI quartiles "
             f"{np.round(np.quantile(xh, [0, 0.25, 0.5, 0.75, 1]), 3).tolist()}")
         t = time.perf_counter()
-        losses = train(model, train_set, args.steps, lr=args.lr, log_every=50, log=log,
-                       time_budget=args.time_budget or None)
+        losses = train(model, train_set, args.steps, lr=args.lr, lr_min=args.lr_min, log_every=50, log=log,
+                       time_budget=args.time_budget or None, augment=args.augment)
         torch.cuda.synchronize()
         report["train"] = {"boxes": len(train_set), "steps": losses[-1][0] + 1, "seconds": time.perf_counter() - t,
                            "loss_first": losses[0][1], "loss_last": losses[-1][1]}
+        if args.calibrate:
+            a, table = calibrate_logit_scale(model, train_set, idx=range(min(args.calibrate, len(train_set))), log=log)
+            report["logit_scale"] = a
+            log(f"output temperature: logit scale {a}")
         del train_set
         torch.cuda.empty_cache()
         if args.out:
             model.save(args.out, **{k: v for k, v in report.items() if k != "train"})
+            log(f"weights written to {args.out}")
+    elif args.calibrate:
+        # calibration on the first training seeds only (the same draws as training)
+        seeds = list(_seed_range(args.train_seeds))
+        specs = surrogate_specs(seeds, seed=7, D=args.D, box_len=args.box_len, deltac=args.deltac)
+        cal_set = BoxSet(specs[:args.calibrate])
+        a, _ = calibrate_logit_scale(model, cal_set, log=log)
+        report["logit_scale"] = a
+        log(f"output temperature: logit scale {a}")
+        del cal_set
+        if args.out:
+            model.save(args.out, **report)
             log(f"weights written to {args.out}")
     eval_set = BoxSet(surrogate_specs(_seed_range(args.eval_seeds), seed=8, D=args.D, box_len=args.box_len,
                                       deltac=args.deltac))

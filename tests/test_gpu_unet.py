@@ -64,14 +64,17 @@ def _conv_call(lib, B, D, cin, cout, x0, w, y0, x1=None, c0=None, mask=None, bia
     lib.call("cm21_unet_conv3_fwd", ctypes.byref(c), lib.stream_handle())
 
 
+# D < 64: one 2 x 2 x 16 tile per block; D = 64 and D >= 128: sliding-window
+# kernel over 8 and 16 planes
 SHAPES = [(16, 16, 16, False), (16, 32, 32, False), (32, 16, 64, True), (32, 32, 16, False), (32, 64, 16, False),
-          (64, 32, 32, True), (64, 64, 16, False), (64, 64, 64, False)]
+          (64, 32, 32, True), (64, 64, 16, False), (64, 64, 64, False), (16, 32, 64, False), (32, 32, 64, False),
+          (64, 32, 64, True), (16, 16, 128, False), (32, 16, 128, True)]
 
 
 @pytest.mark.parametrize("cin,cout,D,split", SHAPES)
 def test_conv3_fwd(gpu, cin, cout, D, split):
     lib = gpu
-    B = 2
+    B = 1 if D >= 128 else 2
     x = _bf((B, D, D, D, cin), 0.7, 1)
     w = _bf((cout, 27, cin), np.sqrt(2.0 / (27 * cin)), 2)
     bias = _f((B, cout) if split else (cout,), 0.1, 3)
@@ -93,7 +96,8 @@ def test_conv3_fwd(gpu, cin, cout, D, split):
         _close_bf16(y1, ref[..., o0:], "y1")
 
 
-@pytest.mark.parametrize("cin,cout,D", [(16, 16, 32), (32, 16, 16), (64, 32, 16), (32, 64, 16)])
+@pytest.mark.parametrize("cin,cout,D", [(16, 16, 32), (32, 16, 16), (64, 32, 16), (32, 64, 16), (16, 16, 64),
+                                        (16, 32, 64), (32, 16, 64)])
 def test_conv3_input_and_weight_gradients(gpu, cin, cout, D):
     """Input gradient = the MFMA conv on the masked output gradient with the
     packed transposed weights; weight/bias gradients from the wgrad kernel;

@@ -242,3 +242,52 @@ def test_batched_workspace_query(cdll):
     assert cdll.cm21_find_bubbles_batched_workspace_bytes(arr, 3, ctypes.byref(prm)) == 0
     assert cdll.cm21_batch_tile_count(128, 0) >= 1 and cdll.cm21_batch_tile_count(100, 0) == -1
     assert cdll.cm21_batch_tile_name(128, 1, 0) is not None
+
+
+def test_batch_manifest_is_reproducible():
+    """The committed 64-box manifest is what its fixed generator draws."""
+    import json
+    bm = pkg("Programs.find_HII_bubbles_batched")
+    assert json.load(open(bm.MANIFEST)) == json.loads(json.dumps(bm.make_manifest()))
+    man = bm.load_manifest()
+    for b in man["boxes"]:
+        assert b["box_len"] == 1.5625 * b["D"] and 10 <= b["zeta"] <= 60 and 2 <= b["sigma_min2"] <= 6
+        assert b["z"] in (7.0, 8.0, 9.0, 10.0)
+    assert [b["seed"] for b in man["boxes"]] == list(range(64))
+
+
+def test_batch_box_parameters():
+    bm = pkg("Programs.find_HII_bubbles_batched")
+    from scipy.special import erfc
+    deltac, ginv, fc = bm.box_parameters({"z": 8.0, "sigma_min2": 4.0})
+    assert deltac == pytest.approx(1.686 / 0.12)
+    assert float(ginv) == pytest.approx(1 / 0.12, rel=1e-6)
+    assert float(fc) == pytest.approx(erfc(deltac / np.sqrt(8.0)), rel=1e-6)
+    assert bm.box_parameters({"deltac": 2.0, "sigma_min2": 1.0})[0] == 2.0
+    assert bm.linear_growth(8.0) == pytest.approx(0.12)
+
+
+def test_batch_tiles_file_names_valid_candidates(cdll):
+    """The committed autotuned tiles name existing candidates of each D."""
+    bm = pkg("Programs.find_HII_bubbles_batched")
+    tiles = bm.load_tiles()
+    assert tiles
+    for key, v in tiles.items():
+        D = int(key.split("_")[0][1:])
+        assert 0 <= v["columns"] < cdll.cm21_batch_tile_count(D, 0)
+        assert 0 <= v["lines"] < cdll.cm21_batch_tile_count(D, 1)
+        assert v["chosen"][0] == cdll.cm21_batch_tile_name(D, 0, v["columns"]).decode()
+
+
+def test_surrogate_specs_and_weights_file():
+    sur = pkg("surrogate")
+    a = sur.surrogate_specs(range(1000, 1010))
+    assert a == sur.surrogate_specs(range(1000, 1010))
+    for s in a:
+        assert 10 <= s["zeta"] <= 60 and 2 <= s["sigma_min2"] <= 6 and s["deltac"] == 4.0 and s["D"] == 128
+    params, meta = sur.UNet3D.load_file(sur.WEIGHTS)
+    shapes = dict(sur.param_shapes())
+    assert set(params) == set(shapes)
+    for k, v in params.items():
+        assert v.shape == shapes[k] and np.all(np.isfinite(v))
+    assert meta["D"] == 128 and meta["deltac"] == 4.0
