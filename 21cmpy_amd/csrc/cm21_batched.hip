@@ -136,10 +136,11 @@ __global__ __launch_bounds__(L* N1) void k_batch_lines_sum(BatchLaunch a) {
     const BatchCell cell = a.cells[box * a.nr + r];
     const float* xrf = reinterpret_cast<const float*>(buf);
     double acc = 0.0;
+    const float inv_denom = 1.0f / cell.denom;
 #pragma unroll 4
     for (int f = t; f < L * D; f += NT) {
         const int i = f / D, rr = f - i * D;
-        acc += (double)fcoll_of(xrf[i * 2 * C::HP + rr], cell.deltac, cell.denom);
+        acc += (double)fcoll_for_sum(xrf[i * 2 * C::HP + rr], cell.deltac, inv_denom);
     }
     acc = block_sum(acc);
     if (t == 0) a.partial[((size_t)box * a.nr + r) * a.nblk + blockIdx.x] = acc;
@@ -568,14 +569,33 @@ int upload(const Prepared& p, void* work, hipStream_t st) {
     return CM21_OK;
 }
 
+int run_group(const Prepared& p, size_t i, const cm21_bubble_params* prm, void* work, hipStream_t gs) {
+    const GroupHost& h = p.lay.groups[i];
+    ProfToken t = prof_begin(CM21_STAGE_R2C, gs);
+    CM21_TRY(run_forward(h, gs, (char*)work));
+    prof_end(t, gs);
+    CM21_TRY(run_transforms(h, p.launches[i], p.col_tile[i], gs));
+    CM21_TRY(run_threshold(h, p.launches[i], p.line_tile[i], prm, gs));
+    return CM21_OK;
+}
+
+// Groups are independent: the first runs on the caller's stream, the others
+// on library helper streams forked after the table upload and joined back at
+// the end, so the small passes of one group overlap the persistent passes of
+// another.
 int run_all(const Prepared& p, const cm21_bubble_params* prm, void* work, hipStream_t st) {
-    for (size_t i = 0; i < p.lay.groups.size(); ++i) {
-        const GroupHost& h = p.lay.groups[i];
-        ProfToken t = prof_begin(CM21_STAGE_R2C, st);
-        CM21_TRY(run_forward(h, st, (char*)work));
-        prof_end(t, st);
-        CM21_TRY(run_transforms(h, p.launches[i], p.col_tile[i], st));
-        CM21_TRY(run_threshold(h, p.launches[i], p.line_tile[i], prm, st));
+    const int G = (int)p.lay.groups.size();
+    if (G == 1) return run_group(p, 0, prm, work, st);
+    SideStreams* ss = nullptr;
+    CM21_TRY(side_streams(&ss));
+    std::lock_guard<std::mutex> lk(ss->mu);
+    const int nside = std::min(G - 1, SideStreams::kStreams);
+    CM21_HIP_CHECK(hipEventRecord(ss->fork, st));
+    for (int k = 0; k < nside; ++k) CM21_HIP_CHECK(hipStreamWaitEvent(ss->s[k], ss->fork, 0));
+    for (int i = 0; i < G; ++i) CM21_TRY(run_group(p, i, prm, work, i == 0 ? st : ss->s[(i - 1) % nside]));
+    for (int k = 0; k < nside; ++k) {
+        CM21_HIP_CHECK(hipEventRecord(ss->join[k], ss->s[k]));
+        CM21_HIP_CHECK(hipStreamWaitEvent(st, ss->join[k], 0));
     }
     return CM21_OK;
 }

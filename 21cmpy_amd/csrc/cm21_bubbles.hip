@@ -320,34 +320,6 @@ extern "C" int cm21_find_bubbles(cm21_plan* plan, const float* delta, float* ion
 }
 
 namespace {
-// Side stream and events of the fused loop, one set per device (created on
-// first use, default priority, kept until the process ends); the mutex
-// serialises host threads that launch loops on the same device.
-struct Overlap {
-    std::mutex mu;
-    hipStream_t side = nullptr;
-    hipEvent_t start = nullptr, rows[2] = {nullptr, nullptr}, lines[2] = {nullptr, nullptr};
-};
-std::mutex g_overlap_mu;
-std::unordered_map<int, Overlap> g_overlap;
-
-int overlap_for_device(Overlap** out) {
-    int dev = 0;
-    CM21_HIP_CHECK(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lk(g_overlap_mu);
-    Overlap& o = g_overlap[dev];
-    if (!o.side) {
-        CM21_HIP_CHECK(hipStreamCreateWithFlags(&o.side, hipStreamNonBlocking));
-        CM21_HIP_CHECK(hipEventCreateWithFlags(&o.start, hipEventDisableTiming));
-        for (int i = 0; i < 2; ++i) {
-            CM21_HIP_CHECK(hipEventCreateWithFlags(&o.rows[i], hipEventDisableTiming));
-            CM21_HIP_CHECK(hipEventCreateWithFlags(&o.lines[i], hipEventDisableTiming));
-        }
-    }
-    *out = &o;
-    return CM21_OK;
-}
-
 int run_loop(cm21_plan* plan, const float* delta, float* ionized, const float* radii, const float* denoms,
              int n_radii, const cm21_bubble_params* prm, const LoopWork& w, hipStream_t st) {
     void* stream = (void*)st;
@@ -369,21 +341,23 @@ int run_loop(cm21_plan* plan, const float* delta, float* ionized, const float* r
         // passes of radius r: they run on a side stream into the other
         // intermediate buffer, joined by events (captured into the same graph
         // when the caller's stream is capturing).
-        Overlap* ov = nullptr;
-        CM21_TRY(overlap_for_device(&ov));
+        SideStreams* ov = nullptr;
+        CM21_TRY(side_streams(&ov));
         std::lock_guard<std::mutex> lk(ov->mu);
-        hipStream_t ss = ov->side;
+        hipStream_t ss = ov->s[0];
+        hipEvent_t* rows_done = ov->ev;       // [2]
+        hipEvent_t* lines_done = ov->ev + 2;  // [2]
         float2* T[2] = {w.filt, w.filt2};
-        CM21_HIP_CHECK(hipEventRecord(ov->start, st));
-        CM21_HIP_CHECK(hipStreamWaitEvent(ss, ov->start, 0));
+        CM21_HIP_CHECK(hipEventRecord(ov->fork, st));
+        CM21_HIP_CHECK(hipStreamWaitEvent(ss, ov->fork, 0));
         for (int r = 0; r < n_radii; ++r) {
             const bool final_step = r == n_radii - 1;
             const int slot = r & 1;
             FusedParams fp = fused_params(D, dk, radii[r], prm->filter_type, prm->ref_fallthrough, scale);
-            if (r >= 2) CM21_HIP_CHECK(hipStreamWaitEvent(ss, ov->lines[slot], 0));
+            if (r >= 2) CM21_HIP_CHECK(hipStreamWaitEvent(ss, lines_done[slot], 0));
             CM21_TRY(fused_radius(*sh, w.spec, T[slot], tw, fp, ss));
-            CM21_HIP_CHECK(hipEventRecord(ov->rows[slot], ss));
-            CM21_HIP_CHECK(hipStreamWaitEvent(st, ov->rows[slot], 0));
+            CM21_HIP_CHECK(hipEventRecord(rows_done[slot], ss));
+            CM21_HIP_CHECK(hipStreamWaitEvent(st, rows_done[slot], 0));
             // Nonlinear input: a sum pass for the FP64 box mean, then the
             // threshold pass recomputes the (cheap) last-axis transform rather
             // than storing and re-reading the smoothed field.
@@ -398,7 +372,7 @@ int run_loop(cm21_plan* plan, const float* delta, float* ionized, const float* r
             LineEpilogue ep = line_epilogue(prm->deltac, denoms[r], prm->zeta, sum, prm->fc_mean, w.mask, ionized,
                                             w.lpart, (double)N);
             CM21_TRY(fused_lines(*sh, final_step ? 2 : 1, T[slot], tw, fp, ep, st));
-            CM21_HIP_CHECK(hipEventRecord(ov->lines[slot], st));
+            CM21_HIP_CHECK(hipEventRecord(lines_done[slot], st));
         }
         return CM21_OK;
     }

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <mutex>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -154,6 +155,25 @@ extern "C" int cm21_c2r(cm21_plan* plan, void* spec_in, float* out, void* stream
 
 // Plan dimension accessor used by the loop driver in other translation units.
 namespace cm21 {
+int side_streams(SideStreams** out) {
+    static std::mutex mu;
+    static std::unordered_map<int, SideStreams> sets;
+    int dev = 0;
+    CM21_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    SideStreams& o = sets[dev];
+    if (!o.fork) {
+        for (int i = 0; i < SideStreams::kStreams; ++i) {
+            CM21_HIP_CHECK(hipStreamCreateWithFlags(&o.s[i], hipStreamNonBlocking));
+            CM21_HIP_CHECK(hipEventCreateWithFlags(&o.join[i], hipEventDisableTiming));
+        }
+        for (auto& e : o.ev) CM21_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        CM21_HIP_CHECK(hipEventCreateWithFlags(&o.fork, hipEventDisableTiming));
+    }
+    *out = &o;
+    return CM21_OK;
+}
+
 int plan_dim(const cm21_plan* p) { return p ? p->D : 0; }
 int plan_batch(const cm21_plan* p) { return p ? p->batch : 0; }
 const float2* plan_twiddles(const cm21_plan* p) { return p ? p->twiddles : nullptr; }

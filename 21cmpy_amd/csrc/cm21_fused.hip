@@ -84,12 +84,13 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
     double acc = 0.0;
     const int lane = t & 63;
     const float* xrf = reinterpret_cast<const float*>(buf);
+    const float inv_denom = 1.0f / ep.denom;
 #pragma unroll 4
     for (int f = t; f < L * D; f += NT) {
         const int i = f / D, r = f - i * D;
         const float delta = xrf[i * 2 * C::HP + r];
         if (MODE == kLineSum) {
-            acc += (double)fcoll_of(delta, ep.deltac, ep.denom);
+            acc += (double)fcoll_for_sum(delta, ep.deltac, inv_denom);
         } else {
             const bool hit = cell_hit(delta, th, ep.deltac, ep.denom, use_scale, scale, ep.inv_zeta);
             const unsigned long long word = smask[f >> 6];

@@ -211,6 +211,14 @@ __device__ inline float fcoll_of(float smoothed, float deltac, float denom) {
     return erfcf(__fdiv_rn(__fsub_rn(deltac, smoothed), denom));
 }
 
+// f_coll for the box-mean sum only: a multiply by the reciprocal of the
+// denominator instead of the correctly rounded division (at most a few ulp
+// per cell; the FP64 mean changes by ~1e-7 relative, far inside the 1e-5
+// decision band).  Decisions use fcoll_of.
+__device__ inline float fcoll_for_sum(float smoothed, float deltac, float inv_denom) {
+    return erfcf((deltac - smoothed) * inv_denom);
+}
+
 // Threshold of the centre-pixel rule in density space.  f = erfcf((deltac -
 // d)/denom) * scale >= 1/zeta holds exactly when d >= d*, with
 // d* = deltac - denom * erfcinv(1/(zeta*scale)), up to the float32 rounding of

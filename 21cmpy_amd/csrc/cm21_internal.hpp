@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 
 #include "cm21.h"
 
@@ -66,6 +67,21 @@ bool prof_any();  // any stage instrumented
 int plan_dim(const ::cm21_plan* p);
 int plan_batch(const ::cm21_plan* p);
 const float2* plan_twiddles(const ::cm21_plan* p);
+
+// Library-owned helper streams of one device (default priority, created on
+// first use, kept until the process ends) and events to fork work off a
+// caller's stream and join it back (the fork/join becomes graph edges when
+// the caller's stream is capturing).  `mu` serialises host threads that
+// launch through the same set.
+struct SideStreams {
+    static constexpr int kStreams = 4;
+    std::mutex mu;
+    hipStream_t s[kStreams] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kStreams] = {};
+    hipEvent_t ev[4] = {};  // free for the loop's own dependencies
+};
+int side_streams(SideStreams** out);
 
 // Drop captured graphs that reference a plan about to be destroyed.
 void forget_plan_graphs(const ::cm21_plan* plan);

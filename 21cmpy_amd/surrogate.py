@@ -348,7 +348,8 @@ class BoxSet:
 
 def box_metrics(field_sur, field_plain, delta, box_len, nbins=40):
     """|dx_HI|, cell agreement (ionized = field >= 0.5) and the P21(k) relative
-    error (max over bins with power above 1e-3 of the peak) of one box."""
+    error (max, and mean, over the bins with power above 1e-3 of the peak) of
+    one box."""
     dt = importlib.import_module(__package__ + ".Programs.delta_T")
     D = field_plain.shape[0]
     xs = 1.0 - float(field_sur.double().mean())
@@ -361,8 +362,11 @@ def box_metrics(field_sur, field_plain, delta, box_len, nbins=40):
         _, _, pk, _ = dt.power_spectrum(dT, box_len, edges, field_sum=s)
         ps.append(pk)
     ok = ps[1] > 1e-3 * max(ps[1].max(), 1e-300)
-    p21 = float(np.max(np.abs(ps[0][ok] - ps[1][ok]) / ps[1][ok])) if ok.any() else 0.0
-    return {"dx_HI": abs(xs - xp), "x_HI_plain": xp, "x_HI_sur": xs, "agreement": agree, "p21_err": p21}
+    rel = np.abs(ps[0][ok] - ps[1][ok]) / ps[1][ok]
+    p21 = float(np.max(rel)) if ok.any() else 0.0
+    p21_mean = float(np.mean(rel)) if ok.any() else 0.0
+    return {"dx_HI": abs(xs - xp), "x_HI_plain": xp, "x_HI_sur": xs, "agreement": agree, "p21_err": p21,
+            "p21_err_mean_bins": p21_mean}
 
 
 def evaluate(model, boxes, idx=None):
@@ -384,9 +388,13 @@ def summarise(metrics, tol_xhi=0.01, tol_p21=0.10):
     dx = np.array([m["dx_HI"] for m in metrics])
     ag = np.array([m["agreement"] for m in metrics])
     p21 = np.array([m["p21_err"] for m in metrics])
+    p21m = np.array([m["p21_err_mean_bins"] for m in metrics])
     return {"boxes": len(metrics), "dx_HI_mean": float(dx.mean()), "dx_HI_max": float(dx.max()),
+            "frac_dx_HI_within_tol": float(np.mean(dx < tol_xhi)),
             "agreement_mean": float(ag.mean()), "p21_err_median": float(np.median(p21)),
-            "frac_within_tol": float(np.mean((dx < tol_xhi) & (p21 < tol_p21)))}
+            "p21_err_mean_bins_median": float(np.median(p21m)),
+            "frac_within_tol": float(np.mean((dx < tol_xhi) & (p21 < tol_p21))),
+            "frac_within_tol_mean_bins": float(np.mean((dx < tol_xhi) & (p21m < tol_p21)))}
 
 
 # ------------------------------------------------------------------ training
