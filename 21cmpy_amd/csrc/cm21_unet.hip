@@ -83,12 +83,13 @@ struct ConvArgs {
     bf16_t* y1;
 };
 
-template <int CIN, int COUT, int TZ, int TY>
-__global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
+template <int CIN, int COUT, int TZ, int TY, int WAVES = 4>
+__global__ __launch_bounds__(64 * WAVES) void k_conv3_mfma(ConvArgs a) {
+    constexpr int NTH = 64 * WAVES;
     constexpr int HZ = TZ + 2, HY = TY + 2, HX = 18, CP = CIN + 8;
-    constexpr int MT = TZ * TY, MW = MT / 4, NT = COUT / 16;
+    constexpr int MT = TZ * TY, MW = MT / WAVES, NT = COUT / 16;
     constexpr int K = 27 * CIN, KC = (K + 31) / 32;
-    static_assert(MT % 4 == 0, "four waves share the M-tiles");
+    static_assert(MT % WAVES == 0, "the waves share the M-tiles");
     static_assert(CIN % 8 == 0 && COUT % 16 == 0, "channel counts");
     __shared__ __attribute__((aligned(16))) bf16_t tile[HZ * HY * HX * CP];
     const int D = a.D;
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
     const int y0 = (blockIdx.y % nyt) * TY, z0 = (blockIdx.y / nyt) * TZ;
     const int b = blockIdx.z;
     const int c1 = CIN - a.c0;
-    for (int i = threadIdx.x; i < HZ * HY * HX * (CIN / 8); i += 256) {
+    for (int i = threadIdx.x; i < HZ * HY * HX * (CIN / 8); i += NTH) {
         const int ch = (i % (CIN / 8)) * 8;
         const int vox = i / (CIN / 8);
         const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
@@ -132,16 +133,13 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
         for (int n = 0; n < NT; ++n)
             bw[n] = __builtin_bit_cast(bf16x8, valid ? *(const u32x4*)(a.w + (size_t)(n * 16 + row) * K + k0) : zero);
     };
-    bf16x8 bcur[NT], bnext[NT];
-    load_b(0, bcur);
-    for (int kc = 0; kc < KC; ++kc) {
-        if (kc + 1 < KC) load_b(kc + 1, bnext);
+    // A fragments of this wave's M-tiles for slice kc from the LDS tile
+    auto load_a = [&](int kc, bf16x8 (&av)[MW]) {
         const int kraw = kc * 32 + grp * 8;
         const bool valid = kraw < K;
         const int k0 = valid ? kraw : 0;
         const int tap = k0 / CIN, ci = k0 % CIN;
         const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-        bf16x8 av[MW];
 #pragma unroll
         for (int m = 0; m < MW; ++m) {
             const int mt = wave * MW + m;
@@ -149,12 +147,25 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
             const int off = (((lz + dz) * HY + (ly + dy)) * HX + (row + dx)) * CP + ci;
             av[m] = __builtin_bit_cast(bf16x8, valid ? *(const u32x4*)(tile + off) : zero);
         }
+    };
+    // software pipeline: the A and B fragments of slice kc+1 load while the
+    // MFMAs of slice kc run
+    bf16x8 bcur[NT], bnext[NT], acur[MW], anext[MW];
+    load_b(0, bcur);
+    load_a(0, acur);
+    for (int kc = 0; kc < KC; ++kc) {
+        if (kc + 1 < KC) {
+            load_b(kc + 1, bnext);
+            load_a(kc + 1, anext);
+        }
 #pragma unroll
         for (int m = 0; m < MW; ++m)
 #pragma unroll
-            for (int n = 0; n < NT; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bcur[n], acc[m][n], 0, 0, 0);
+            for (int n = 0; n < NT; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[m], bcur[n], acc[m][n], 0, 0, 0);
 #pragma unroll
         for (int n = 0; n < NT; ++n) bcur[n] = bnext[n];
+#pragma unroll
+        for (int m = 0; m < MW; ++m) acur[m] = anext[m];
     }
     const int o1 = COUT - a.o0;
 #pragma unroll
@@ -952,10 +963,10 @@ __global__ void k_adamw(float* __restrict__ p, const float* __restrict__ g, floa
 // ------------------------------------------------------------------ dispatch
 
 using ConvFn = void (*)(const ConvArgs&, hipStream_t);
-template <int CIN, int COUT, int TZ, int TY>
+template <int CIN, int COUT, int TZ, int TY, int WAVES = 4>
 void launch_conv(const ConvArgs& a, hipStream_t st) {
     const dim3 grid(a.D / 16, (a.D / TZ) * (a.D / TY), a.B);
-    k_conv3_mfma<CIN, COUT, TZ, TY><<<grid, 256, 0, st>>>(a);
+    k_conv3_mfma<CIN, COUT, TZ, TY, WAVES><<<grid, 64 * WAVES, 0, st>>>(a);
 }
 template <int CIN, int COUT, int TY, int NZ>
 void launch_slide(const ConvArgs& a, hipStream_t st) {
@@ -1056,13 +1067,12 @@ extern "C" int cm21_unet_conv3_fwd(const cm21_unet_conv* c, void* stream) {
     a.y1 = (bf16_t*)c->y1;
     hipStream_t st = (hipStream_t)stream;
     CM21_REQUIRE(c->algo >= 0 && c->algo <= 2, "cm21_unet_conv3_fwd: algo=%d", c->algo);
-    CM21_REQUIRE(c->algo != 2 || c->D >= 64, "cm21_unet_conv3_fwd: the sliding form needs D >= 64");
+    CM21_REQUIRE(c->algo < 2 || c->D >= 64, "cm21_unet_conv3_fwd: algo %d needs D >= 64", c->algo);
     // automatic choice from per-layer timings on the MI355X: the sliding
     // window wins for 16 -> 16 channels at D >= 128, one tile per block
     // elsewhere
     const int algo = c->algo ? c->algo : ((c->cin == 16 && c->cout == 16 && c->D >= 128) ? 2 : 1);
-    ConvFn fn = c->D < 64 ? s->tile_small
-                          : (algo == 1 ? s->tile_big : (c->D >= 128 ? s->slide128 : s->slide64));
+    ConvFn fn = c->D < 64 ? s->tile_small : (algo == 1 ? s->tile_big : (c->D >= 128 ? s->slide128 : s->slide64));
     fn(a, st);
     CM21_LAUNCH_CHECK("k_conv3_mfma");
     return CM21_OK;

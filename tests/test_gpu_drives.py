@@ -1,0 +1,40 @@
+"""The redshift scroll (Programs/Drives.py): one initialization, the density
+at each redshift, and one batched find_HII_bubbles call for all redshifts,
+checked against the stage functions called one by one."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import pkg
+
+pytestmark = pytest.mark.gpu
+
+
+def test_zscroll_matches_stage_by_stage(gpu):
+    drv = pkg("Programs.Drives")
+    init = pkg("Programs.initialize")
+    pert = pkg("Programs.perturb_field")
+    fb = pkg("Programs.find_HII_bubbles")
+    bm = pkg("Programs.find_HII_bubbles_batched")
+    DIM, D, L = 128, 64, 100.0
+    # sigma_min^2 in z=0 units such that the barrier 1.686/D(z) is crossed in part of the box
+    out = drv.drive_zscroll_noTs(12.0, 6.0, -2.0, DIM=DIM, HII_DIM=D, BOX_LEN=L, seed=3, zeta=30.0, sigma_min2=40.0,
+                                 quiet=True)
+    assert list(out["z"]) == [12.0, 10.0, 8.0]
+    ics = init.init(DIM=DIM, HII_DIM=D, BOX_LEN=L, seed=3, save=False)
+    d_init = bm.linear_growth(300.0)
+    for i, z in enumerate(out["z"]):
+        upd = pert.evolve_zeldovich(z, ics["deltax"], ics["v"], bm.linear_growth(z), d_init, L, "cic")
+        # the CIC deposit uses float atomics: equal up to the order of additions
+        assert torch.allclose(upd, out["density"][i], rtol=0, atol=1e-5)
+        single = fb.conv_bubbles(out["density"][i], {"BoxSize": L, "HIIdim": D}, LE=False, quiet=True,
+                                 deltac=1.686 / bm.linear_growth(z), sigma_min2=40.0, zeta=30.0).cpu().numpy()
+        got = out["ionized"][i].cpu().numpy()
+        print(f"z={z}: x_HI {out['x_HI'][i]:.4f}", flush=True)
+        assert np.mean((got == 1) != (single == 1)) <= 1e-5
+        both = (got < 1) & (single < 1)
+        assert np.allclose(got[both], single[both], rtol=0, atol=1e-5)
+        K, k_ave, ps = out["P21"][i]
+        assert len(ps) == 40 and np.all(np.isfinite(ps))
+    # later redshifts are more ionized
+    assert out["x_HI"][0] >= out["x_HI"][1] >= out["x_HI"][2]

@@ -44,6 +44,7 @@ def _fp(t):
 
 def _close_bf16(got, ref, what):
     got, ref = _d(got), ref.to(torch.float64)
+    assert bool(torch.isfinite(got).all()), f"{what}: non-finite values"
     tol = HALF_ULP * ref.abs() + 1e-5 * float(ref.abs().max()) + 1e-30
     bad = ((got - ref).abs() > tol)
     assert not bool(bad.any()), f"{what}: {int(bad.sum())} of {bad.numel()} outside, max err " \
@@ -57,15 +58,16 @@ def _close_f32(got, ref, what, rel=1e-3):
 
 
 def _conv_call(lib, B, D, cin, cout, x0, w, y0, x1=None, c0=None, mask=None, bias=None, relu=True, y1=None, o0=None,
-               per_sample=False):
+               per_sample=False, algo=0):
     c = lib.UnetConv(B=B, D=D, cin=cin, cout=cout, c0=c0 or cin, o0=o0 or cout, x0=_vp(x0), x1=_vp(x1),
                      mask=_vp(mask), w=_vp(w), bias=_fp(bias), bias_per_sample=1 if per_sample else 0,
-                     relu=1 if relu else 0, y0=_vp(y0), y1=_vp(y1))
+                     relu=1 if relu else 0, y0=_vp(y0), y1=_vp(y1), algo=algo)
     lib.call("cm21_unet_conv3_fwd", ctypes.byref(c), lib.stream_handle())
 
 
-# D < 64: one 2 x 2 x 16 tile per block; D = 64 and D >= 128: sliding-window
-# kernel over 8 and 16 planes
+# D < 64: one 2 x 2 x 16 tile per block; D >= 64: both the 4 x 4 x 16 (or
+# 4 x 2 x 16) tile per block (algo 1) and the sliding window over 8 (D = 64)
+# or 16 planes (D >= 128, algo 2)
 SHAPES = [(16, 16, 16, False), (16, 32, 32, False), (32, 16, 64, True), (32, 32, 16, False), (32, 64, 16, False),
           (64, 32, 32, True), (64, 64, 16, False), (64, 64, 64, False), (16, 32, 64, False), (32, 32, 64, False),
           (64, 32, 64, True), (16, 16, 128, False), (32, 16, 128, True)]
@@ -83,17 +85,20 @@ def test_conv3_fwd(gpu, cin, cout, D, split):
     x0, x1 = x[..., :c0].contiguous(), (x[..., c0:].contiguous() if split else None)
     y0 = torch.empty((B, D, D, D, o0), dtype=torch.bfloat16, device="cuda")
     y1 = torch.empty((B, D, D, D, cout - o0), dtype=torch.bfloat16, device="cuda") if o0 < cout else None
-    print(f"conv fwd cin={cin} cout={cout} D={D} split={split}", flush=True)
-    _conv_call(lib, B, D, cin, cout, x0, w, y0, x1=x1, c0=c0, bias=bias, y1=y1, o0=o0, per_sample=split)
-    torch.cuda.synchronize()
     ref = []
     for b in range(B):
         bb = _d(bias[b]) if split else _d(bias)
         ref.append(unet.conv3(_d(x[b:b + 1]), _d(w), bb)[0])
     ref = torch.stack(ref)
-    _close_bf16(y0, ref[..., :o0], "y0")
-    if y1 is not None:
-        _close_bf16(y1, ref[..., o0:], "y1")
+    for algo in ((1, 2) if D >= 64 else (0,)):
+        print(f"conv fwd cin={cin} cout={cout} D={D} split={split} algo={algo}", flush=True)
+        y0.fill_(float("nan"))
+        _conv_call(lib, B, D, cin, cout, x0, w, y0, x1=x1, c0=c0, bias=bias, y1=y1, o0=o0, per_sample=split,
+                   algo=algo)
+        torch.cuda.synchronize()
+        _close_bf16(y0, ref[..., :o0], f"y0 algo {algo}")
+        if y1 is not None:
+            _close_bf16(y1, ref[..., o0:], f"y1 algo {algo}")
 
 
 @pytest.mark.parametrize("cin,cout,D", [(16, 16, 32), (32, 16, 16), (64, 32, 16), (32, 64, 16), (16, 16, 64),
