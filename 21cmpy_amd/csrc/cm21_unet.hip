@@ -12,20 +12,24 @@
 //
 //   3^3 convolution, forward:  implicit GEMM on v_mfma_f32_16x16x32_bf16.  A
 //     block computes TZ x TY x 16 output voxels for all output channels: the
-//     input tile with a one-voxel halo sits in LDS; for each 32-wide slice of
-//     K = 27 * cin (tap-major, channel-minor) every lane reads 8 consecutive
-//     channels of one voxel (A, 16 bytes from LDS) and 8 consecutive K of one
-//     output channel (B, 16 bytes of the bf16 weights); the 16 x 16 x 32
-//     product accumulates in registers.
+//     input tile with a one-voxel halo sits in LDS (or a ring of z-planes for
+//     the sliding form); for each 32-wide slice of K = 27 * cin (tap-major,
+//     channel-minor) every lane reads 8 consecutive K of one output channel
+//     (the A operand, 16 bytes of the bf16 weights) and 8 consecutive
+//     channels of one voxel (B, 16 bytes from LDS); the 16 x 16 x 32 product
+//     accumulates in registers as channels x voxels, so each lane stores 4
+//     consecutive channels of one voxel.
 //   3^3 convolution, input gradient: the same kernel on the output gradient
 //     (masked by the ReLU of the forward output) with the weights transposed
 //     and spatially flipped.
 //   3^3 convolution, weight gradient: fp32 VALU over voxel slabs with the
 //     input halo tile and the masked output gradient in LDS; per-block
 //     partials reduced in a fixed order.
-//   2x2x2 average pooling, 2x2x2 stride-2 transposed convolution, the 1^3
-//   head with sigmoid + binary cross entropy, the input layer (density plus
-//   two constant conditioning channels, folded into a per-sample bias), and
+//   Input layer (density plus two constant conditioning channels, the
+//   latter folded into a per-sample bias): 3^3 taps on the fp32 matrix cores
+//   (v_mfma_f32_16x16x4f32).
+//   2x2x2 average pooling, 2x2x2 stride-2 transposed convolution (bf16
+//   matrix cores), the 1^3 head with sigmoid + binary cross entropy, and
 //   AdamW: fp32 VALU.
 #include <cmath>
 #include <cstring>
@@ -43,12 +47,15 @@ typedef unsigned short bf16_t;
 
 __device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((unsigned)h << 16); }
 
-__device__ __forceinline__ bf16_t f2bf(float f) {
-    unsigned u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16_t)0x7fc0;  // NaN
-    u += 0x7fffu + ((u >> 16) & 1u);                             // round to nearest even
-    return (bf16_t)(u >> 16);
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+// two floats to two bf16 packed low/high, round to nearest even (one
+// v_cvt_pk_bf16_f32)
+__device__ __forceinline__ unsigned f2bf2(float lo, float hi) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(floatx2{lo, hi}, bf16x2));
 }
+__device__ __forceinline__ bf16_t f2bf(float f) { return (bf16_t)(f2bf2(f, 0.0f) & 0xffffu); }
 
 __device__ __forceinline__ int wrap(int i, int D) { return i < 0 ? i + D : (i >= D ? i - D : i); }
 
@@ -83,111 +90,231 @@ struct ConvArgs {
     bf16_t* y1;
 };
 
-template <int CIN, int COUT, int TZ, int TY, int WAVES = 4>
-__global__ __launch_bounds__(64 * WAVES) void k_conv3_mfma(ConvArgs a) {
-    constexpr int NTH = 64 * WAVES;
-    constexpr int HZ = TZ + 2, HY = TY + 2, HX = 18, CP = CIN + 8;
-    constexpr int MT = TZ * TY, MW = MT / WAVES, NT = COUT / 16;
-    constexpr int K = 27 * CIN, KC = (K + 31) / 32;
-    static_assert(MT % WAVES == 0, "the waves share the M-tiles");
-    static_assert(CIN % 8 == 0 && COUT % 16 == 0, "channel counts");
+// The MFMA takes the weights as its A operand and the activations as its B
+// operand, so that lane (row, grp) ends with output channels 4 grp .. 4 grp+3
+// of voxel `row` of the 16-voxel line and writes them as one 8-byte store.
+//
+// K = 27 * cin runs tap-major, channel-minor, padded to a multiple of 32.
+// Lane group grp of slice kc covers k = 32 kc + 8 grp .. +7: for cin >= 32
+// the slice lies inside one tap, for cin = 16 groups 0-1 take tap 2 kc and
+// groups 2-3 tap 2 kc + 1 (tap 27 is the padding, read as zeros).  After the
+// slice loop is unrolled, the tap offsets are compile-time constants and a
+// lane only selects between two of them.
+template <int CIN>
+struct SliceLanes {
+    bool hi;  // cin = 16 and lane group 2 or 3
+    int ci;   // the lane's first channel inside its tap (for cin >= 32 plus the slice's channel base)
+    __device__ explicit SliceLanes(int grp) : hi(CIN == 16 && grp >= 2), ci(CIN == 16 ? (grp & 1) * 8 : grp * 8) {}
+    static constexpr int tap(int kc) { return kc * 32 / CIN; }
+    static constexpr int cbase(int kc) { return CIN == 16 ? 0 : kc * 32 % CIN; }
+    // false for the padding lanes of the last slice
+    __device__ bool valid(int kc) const { return !(hi && tap(kc) + 1 >= 27); }
+};
+
+// element offset of tap t = 9 dz + 3 dy + dx in a box of HY x HX voxels per
+// plane, CP bf16 per voxel; in_plane leaves dz out
+template <int HY, int HX, int CP>
+__device__ __forceinline__ constexpr int tap_off(int t) {
+    return (((t / 9) * HY + (t / 3) % 3) * HX + t % 3) * CP;
+}
+template <int HX, int CP>
+__device__ __forceinline__ constexpr int in_plane(int t) {
+    return (((t / 3) % 3) * HX + t % 3) * CP;
+}
+
+// weight fragment (slice j of input-channel chunk c, N tile n) of this lane:
+// w[16 n + row][k .. k+7] with k = tap * cin + channel; wl = w + row * K +
+// weight_lane (for chunks of 16 channels, lane groups 2-3 sit one tap
+// further).  Zeros for the padding lanes (their address is clamped into the
+// row).
+template <int CIN, int CH>
+__device__ __forceinline__ int weight_lane(int grp) {
+    return CH == 16 ? (grp >= 2 ? CIN : 0) + (grp & 1) * 8 : grp * 8;
+}
+template <int CIN, int CH, int K>
+__device__ __forceinline__ bf16x8 weight_frag(const bf16_t* wl, const SliceLanes<CH>& sl, int c, int j, int n) {
+    const bool ok = sl.valid(j);
+    const int kofs =
+        (CH == 16 ? 2 * j * CIN : SliceLanes<CH>::tap(j) * CIN + SliceLanes<CH>::cbase(j)) + c * CH;
+    const u32x4 v = *(const u32x4*)(wl + n * 16 * K + (ok ? kofs : 0));
+    const u32x4 zero = {0u, 0u, 0u, 0u};
+    return __builtin_bit_cast(bf16x8, ok ? v : zero);
+}
+
+// output channels 16 n + 4 grp .. +3 of voxel vrow + row (vrow: first voxel
+// of the line); N tiles below o0 go to y0, the others to y1
+template <int COUT>
+__device__ __forceinline__ void store_quad(const ConvArgs& a, size_t vrow, int row, int n, int grp, floatx4 v) {
+    if (a.relu) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = fmaxf(v[i], 0.0f);
+    }
+    u32x2 q;
+    q[0] = f2bf2(v[0], v[1]);
+    q[1] = f2bf2(v[2], v[3]);
+    if (n * 16 < a.o0) {
+        bf16_t* p = a.y0 + vrow * a.o0;
+        *(u32x2*)(p + row * a.o0 + n * 16 + grp * 4) = q;
+    } else {
+        const int o1 = COUT - a.o0;
+        bf16_t* p = a.y1 + vrow * o1;
+        *(u32x2*)(p + row * o1 + n * 16 - a.o0 + grp * 4) = q;
+    }
+}
+
+// bias of output channels 16 n + 4 grp .. +3
+template <int NT>
+__device__ __forceinline__ void load_bias(const ConvArgs& a, int b, int grp, floatx4 (&bv)[NT]) {
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+        bv[n] = a.bias ? *(const floatx4*)(a.bias + b * a.bias_stride + n * 16 + grp * 4) : floatx4{0.f, 0.f, 0.f, 0.f};
+}
+
+// One halo tile per block: TZ x TY lines of 16 voxels, input with a one-voxel
+// halo in LDS; each of the four waves owns MW lines of one z-plane.  With
+// CH < CIN the tile holds CH input channels at a time and the K loop walks
+// the channel chunks: a smaller tile, so more lines per block and more MFMAs
+// per weight fragment fetched from L2.
+template <int CIN, int COUT, int TZ, int TY, int CH = CIN>
+__global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
+    constexpr int NCH = CIN / CH;
+    constexpr int HZ = TZ + 2, HY = TY + 2, HX = 18, CP = CH + 8;
+    constexpr int MT = TZ * TY, MW = MT / 4, NT = COUT / 16;
+    constexpr int K = 27 * CIN, KC = (27 * CH + 31) / 32;  // K slices per chunk
+    constexpr int CPR = HX * (CH / 8);                      // 16-byte chunks per halo row
+    constexpr int NR = HZ * HY, NP = (CPR + 63) / 64, RW = NR / 4;
+    static_assert(CIN % CH == 0 && (CH == 16 || CH % 32 == 0), "channel chunks");
+    static_assert(MT % 4 == 0 && TY % MW == 0, "a wave's lines lie in one z-plane");
+    static_assert(NR % 4 == 0, "the waves share the halo rows");
+    static_assert(COUT % 16 == 0, "channel counts");
     __shared__ __attribute__((aligned(16))) bf16_t tile[HZ * HY * HX * CP];
     const int D = a.D;
     const int nyt = D / TY;
     const int x0 = blockIdx.x * 16;
     const int y0 = (blockIdx.y % nyt) * TY, z0 = (blockIdx.y / nyt) * TZ;
     const int b = blockIdx.z;
-    const int c1 = CIN - a.c0;
-    for (int i = threadIdx.x; i < HZ * HY * HX * (CIN / 8); i += NTH) {
-        const int ch = (i % (CIN / 8)) * 8;
-        const int vox = i / (CIN / 8);
-        const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
-        const size_t g = (((size_t)b * D + wrap(z0 + hz - 1, D)) * D + wrap(y0 + hy - 1, D)) * D + wrap(x0 + hx - 1, D);
-        u32x4 v;
-        if (ch < a.c0) {
-            v = *(const u32x4*)(a.x0 + g * a.c0 + ch);
-            if (a.mask) v = mask8(v, *(const u32x4*)(a.mask + g * a.c0 + ch));
-        } else {
-            v = *(const u32x4*)(a.x1 + g * c1 + (ch - a.c0));
-        }
-        *(u32x4*)(tile + vox * CP + ch) = v;
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c0 = a.c0, c1 = CIN - c0;
+    const size_t vb = (size_t)b * D * D * D;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int row = lane & 15, grp = lane >> 4;
+    const u32x4 zero = {0u, 0u, 0u, 0u};
+
+    // halo tile of input channels cb .. cb+CH-1: halo rows (hz, hy) go to the
+    // waves in turn; lane p < CPR of a wave loads 16-byte chunk p of the row
+    // (voxel hx = p / (CH/8), channels ch .. ch+7), so only the row offset
+    // changes between the rows.  All rows are in flight before the LDS
+    // writes; lanes past CPR load and write a copy of chunk CPR-1.
+    auto load_tile = [&](int cb) {
+        const bf16_t* lsrc[NP];
+        const bf16_t* lmask[NP];
+        int lmul[NP], llds[NP];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const int p = min(lane + 64 * j, CPR - 1);
+            const int hx = p / (CH / 8), ch = (p % (CH / 8)) * 8, gch = cb + ch;
+            const unsigned gx = wrap(x0 + hx - 1, D);
+            if (gch < c0) {
+                lsrc[j] = a.x0 + (vb + gx) * c0 + gch;
+                lmul[j] = c0;
+            } else {
+                lsrc[j] = a.x1 + (vb + gx) * c1 + (gch - c0);
+                lmul[j] = c1;
+            }
+            lmask[j] = a.mask ? a.mask + (vb + gx) * c0 + gch : nullptr;
+            llds[j] = hx * CP + ch;
+        }
+        auto row_off = [&](int it) -> unsigned {
+            const int r = wave + 4 * it;
+            const int hz = r / HY, hy = r - hz * HY;
+            return ((unsigned)wrap(z0 + hz - 1, D) * D + wrap(y0 + hy - 1, D)) * D;
+        };
+        if (!a.mask) {
+            u32x4 v[RW][NP];
+#pragma unroll
+            for (int it = 0; it < RW; ++it) {
+                const unsigned rowv = row_off(it);
+#pragma unroll
+                for (int j = 0; j < NP; ++j) v[it][j] = *(const u32x4*)(lsrc[j] + (size_t)rowv * lmul[j]);
+            }
+#pragma unroll
+            for (int it = 0; it < RW; ++it)
+#pragma unroll
+                for (int j = 0; j < NP; ++j) *(u32x4*)(tile + (wave + 4 * it) * HX * CP + llds[j]) = v[it][j];
+        } else {
+            for (int it = 0; it < RW; ++it) {
+                const unsigned rowv = row_off(it);
+#pragma unroll
+                for (int j = 0; j < NP; ++j) {
+                    const u32x4 v = *(const u32x4*)(lsrc[j] + (size_t)rowv * lmul[j]);
+                    const u32x4 m = *(const u32x4*)(lmask[j] + (size_t)rowv * c0);
+                    *(u32x4*)(tile + (wave + 4 * it) * HX * CP + llds[j]) = mask8(v, m);
+                }
+            }
+        }
+    };
+
+    const SliceLanes<CH> sl(grp);
+    const int mt0 = wave * MW, lz = mt0 / TY, ly0 = mt0 % TY;
+    const bf16_t* abase = tile + ((lz * HY + ly0) * HX + row) * CP + sl.ci;
+    const bf16_t* wl = a.w + (size_t)row * K + weight_lane<CIN, CH>(grp);
+    auto load_b = [&](int c, int j, bf16x8 (&bw)[NT]) {
+#pragma unroll
+        for (int n = 0; n < NT; ++n) bw[n] = weight_frag<CIN, CH, K>(wl, sl, c, j, n);
+    };
+    auto load_a = [&](int j, bf16x8 (&av)[MW]) {
+        const int t = SliceLanes<CH>::tap(j);
+        const bool ok = sl.valid(j);
+        const int off = (sl.hi && ok) ? tap_off<HY, HX, CP>(t + 1) : tap_off<HY, HX, CP>(t) + SliceLanes<CH>::cbase(j);
+#pragma unroll
+        for (int m = 0; m < MW; ++m) {
+            const u32x4 v = *(const u32x4*)(abase + off + m * HX * CP);
+            av[m] = __builtin_bit_cast(bf16x8, ok ? v : zero);
+        }
+    };
     floatx4 acc[MW][NT];
 #pragma unroll
     for (int m = 0; m < MW; ++m)
 #pragma unroll
         for (int n = 0; n < NT; ++n) acc[m][n] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-    const u32x4 zero = {0u, 0u, 0u, 0u};
-    // K is padded to a multiple of 32 with zeros; addresses of the padding are
-    // clamped to tap 0 so that every access stays inside the tile and the
-    // weight array.  The weight fragments of slice kc+1 are fetched while the
-    // MFMAs of slice kc run.
-    auto load_b = [&](int kc, bf16x8 (&bw)[NT]) {
-        const int kraw = kc * 32 + grp * 8;
-        const bool valid = kraw < K;
-        const int k0 = valid ? kraw : 0;
+    // software pipeline: the weight fragments (from L2) run PD slices ahead
+    // in a ring (more when a slice has few MFMAs to cover the latency), the
+    // LDS fragments one slice ahead
+    constexpr int PDC = 8 / NT < 2 ? 2 : (8 / NT > 6 ? 6 : 8 / NT);  // ring of at most 32 VGPRs
+    constexpr int PDW = 24 / (MW * NT) < 2 ? 2 : (24 / (MW * NT) > PDC ? PDC : 24 / (MW * NT));
+    constexpr int PD = KC < PDW ? KC : PDW;
 #pragma unroll
-        for (int n = 0; n < NT; ++n)
-            bw[n] = __builtin_bit_cast(bf16x8, valid ? *(const u32x4*)(a.w + (size_t)(n * 16 + row) * K + k0) : zero);
-    };
-    // A fragments of this wave's M-tiles for slice kc from the LDS tile
-    auto load_a = [&](int kc, bf16x8 (&av)[MW]) {
-        const int kraw = kc * 32 + grp * 8;
-        const bool valid = kraw < K;
-        const int k0 = valid ? kraw : 0;
-        const int tap = k0 / CIN, ci = k0 % CIN;
-        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+    for (int c = 0; c < NCH; ++c) {
+        bf16x8 bq[PD][NT], acur[MW], anext[MW];
 #pragma unroll
-        for (int m = 0; m < MW; ++m) {
-            const int mt = wave * MW + m;
-            const int lz = mt / TY, ly = mt % TY;
-            const int off = (((lz + dz) * HY + (ly + dy)) * HX + (row + dx)) * CP + ci;
-            av[m] = __builtin_bit_cast(bf16x8, valid ? *(const u32x4*)(tile + off) : zero);
+        for (int j = 0; j < PD; ++j) load_b(c, j, bq[j]);  // in flight during the tile load
+        if (c > 0) __syncthreads();                       // every wave is done with the previous chunk
+        load_tile(c * CH);
+        __syncthreads();
+        load_a(0, acur);
+#pragma unroll
+        for (int j = 0; j < KC; ++j) {
+            if (j + 1 < KC) load_a(j + 1, anext);
+#pragma unroll
+            for (int m = 0; m < MW; ++m)
+#pragma unroll
+                for (int n = 0; n < NT; ++n)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[j % PD][n], acur[m], acc[m][n], 0, 0, 0);
+            if (j + PD < KC) load_b(c, j + PD, bq[j % PD]);
+#pragma unroll
+            for (int m = 0; m < MW; ++m) acur[m] = anext[m];
+            // keep each slice's loads in its own slice: the scheduler would
+            // otherwise sink them next to their use and expose the L2 latency
+            __builtin_amdgcn_sched_barrier(0);
         }
-    };
-    // software pipeline: the A and B fragments of slice kc+1 load while the
-    // MFMAs of slice kc run
-    bf16x8 bcur[NT], bnext[NT], acur[MW], anext[MW];
-    load_b(0, bcur);
-    load_a(0, acur);
-    for (int kc = 0; kc < KC; ++kc) {
-        if (kc + 1 < KC) {
-            load_b(kc + 1, bnext);
-            load_a(kc + 1, anext);
-        }
-#pragma unroll
-        for (int m = 0; m < MW; ++m)
-#pragma unroll
-            for (int n = 0; n < NT; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[m], bcur[n], acc[m][n], 0, 0, 0);
-#pragma unroll
-        for (int n = 0; n < NT; ++n) bcur[n] = bnext[n];
-#pragma unroll
-        for (int m = 0; m < MW; ++m) acur[m] = anext[m];
     }
-    const int o1 = COUT - a.o0;
+    floatx4 bv[NT];
+    load_bias<NT>(a, b, grp, bv);
 #pragma unroll
     for (int m = 0; m < MW; ++m) {
-        const int mt = wave * MW + m;
-        const int z = z0 + mt / TY, y = y0 + mt % TY;
+        const size_t vrow = vb + ((size_t)(z0 + lz) * D + (y0 + ly0 + m)) * D + x0;
 #pragma unroll
-        for (int n = 0; n < NT; ++n) {
-            const int co = n * 16 + row;
-            const float bv = a.bias ? a.bias[b * a.bias_stride + co] : 0.0f;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int x = x0 + grp * 4 + i;
-                float v = acc[m][n][i] + bv;
-                if (a.relu) v = fmaxf(v, 0.0f);
-                const size_t vox = (((size_t)b * D + z) * D + y) * D + x;
-                if (co < a.o0)
-                    a.y0[vox * a.o0 + co] = f2bf(v);
-                else
-                    a.y1[vox * o1 + (co - a.o0)] = f2bf(v);
-            }
-        }
+        for (int n = 0; n < NT; ++n) store_quad<COUT>(a, vrow, row, n, grp, acc[m][n] + bv[n]);
     }
 }
 
@@ -215,13 +342,17 @@ __global__ __launch_bounds__(256) void k_conv3_slide(ConvArgs a) {
     const int x0 = blockIdx.x * 16;
     const int y0 = (blockIdx.y % nyt) * TY, z0 = (blockIdx.y / nyt) * NZ;
     const int b = blockIdx.z;
-    const int c1 = CIN - a.c0;
+    const int c0 = a.c0, c1 = CIN - c0;
     const int t = threadIdx.x;
     const u32x4 zero = {0u, 0u, 0u, 0u};
+    const size_t vb = (size_t)b * D * D * D;
+    const bf16_t* xs0 = a.x0 + vb * c0;
+    const bf16_t* xs1 = c1 ? a.x1 + vb * c1 : nullptr;
+    const bf16_t* ms = a.mask ? a.mask + vb * c0 : nullptr;
 
     // global -> registers for halo plane hz (global z = z0 - 1 + hz)
     auto fetch = [&](int hz, u32x4 (&pf)[NPF]) {
-        const int gz = wrap(z0 - 1 + hz, D);
+        const unsigned gzD = (unsigned)wrap(z0 - 1 + hz, D) * D;
 #pragma unroll
         for (int q = 0; q < NPF; ++q) {
             const int i = t + q * 256;
@@ -230,12 +361,12 @@ __global__ __launch_bounds__(256) void k_conv3_slide(ConvArgs a) {
                 const int ch = (i % (CIN / 8)) * 8;
                 const int vox = i / (CIN / 8);
                 const int hx = vox % HX, hy = vox / HX;
-                const size_t g = (((size_t)b * D + gz) * D + wrap(y0 + hy - 1, D)) * D + wrap(x0 + hx - 1, D);
-                if (ch < a.c0) {
-                    v = *(const u32x4*)(a.x0 + g * a.c0 + ch);
-                    if (a.mask) v = mask8(v, *(const u32x4*)(a.mask + g * a.c0 + ch));
+                const unsigned g = (gzD + wrap(y0 + hy - 1, D)) * D + wrap(x0 + hx - 1, D);
+                if (ch < c0) {
+                    v = *(const u32x4*)(xs0 + (size_t)g * c0 + ch);
+                    if (ms) v = mask8(v, *(const u32x4*)(ms + (size_t)g * c0 + ch));
                 } else {
-                    v = *(const u32x4*)(a.x1 + g * c1 + (ch - a.c0));
+                    v = *(const u32x4*)(xs1 + (size_t)g * c1 + (ch - c0));
                 }
             }
             pf[q] = v;
@@ -260,21 +391,22 @@ __global__ __launch_bounds__(256) void k_conv3_slide(ConvArgs a) {
             store(hz, pf);
         }
     }
-    const int lane = t & 63, wave = t >> 6;
+    const int lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
     const int row = lane & 15, grp = lane >> 4;
-    auto b_frag = [&](int kc, int n) -> bf16x8 {
-        const int kraw = kc * 32 + grp * 8;
-        return __builtin_bit_cast(bf16x8, kraw < K ? *(const u32x4*)(a.w + (size_t)(n * 16 + row) * K + kraw) : zero);
-    };
+    const SliceLanes<CIN> sl(grp);
+    const bf16_t* wl = a.w + (size_t)row * K + weight_lane<CIN, CIN>(grp);
+    const int lane_off = ((wave * MW) * HX + row) * CP + sl.ci;
     bf16x8 bhold[HOLD ? KC : 1][NT];
     if constexpr (HOLD) {
 #pragma unroll
         for (int kc = 0; kc < KC; ++kc)
 #pragma unroll
-            for (int n = 0; n < NT; ++n) bhold[kc][n] = b_frag(kc, n);
+            for (int n = 0; n < NT; ++n) bhold[kc][n] = weight_frag<CIN, CIN, K>(wl, sl, 0, kc, n);
     }
+    floatx4 bv[NT];
+    load_bias<NT>(a, b, grp, bv);
     __syncthreads();
-    const int o1 = COUT - a.o0;
     for (int oz = 0; oz < NZ; ++oz) {
         u32x4 pf[NPF];
         const bool more = oz + 3 <= NZ + 1;
@@ -284,65 +416,54 @@ __global__ __launch_bounds__(256) void k_conv3_slide(ConvArgs a) {
         for (int m = 0; m < MW; ++m)
 #pragma unroll
             for (int n = 0; n < NT; ++n) acc[m][n] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        // ring slots of the three planes this output plane reads
+        const int s0 = (oz & 3) * PLANE, s1 = ((oz + 1) & 3) * PLANE, s2 = ((oz + 2) & 3) * PLANE;
+        auto slot_of = [&](int dz) { return dz == 0 ? s0 : (dz == 1 ? s1 : s2); };
         // one K slice: A fragments of this wave's lines from the ring, MFMAs
         auto slice = [&](int kc, const bf16x8 (&bw)[NT]) {
-            const int kraw = kc * 32 + grp * 8;
-            const bool valid = kraw < K;
-            const int k0 = valid ? kraw : 0;
-            const int tap = k0 / CIN, ci = k0 % CIN;
-            const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-            const bf16_t* slot = ring + ((oz + dz) & 3) * PLANE;
+            const int tp = SliceLanes<CIN>::tap(kc);
+            const bool ok = sl.valid(kc);
+            const int lo = slot_of(tp / 9) + in_plane<HX, CP>(tp) + SliceLanes<CIN>::cbase(kc);
+            const int up = slot_of(((tp + 1) / 9) % 3) + in_plane<HX, CP>(tp + 1);
+            const int off = (sl.hi && ok) ? up : lo;
             bf16x8 av[MW];
 #pragma unroll
             for (int m = 0; m < MW; ++m) {
-                const int ly = wave * MW + m;
-                av[m] = __builtin_bit_cast(bf16x8, valid ? *(const u32x4*)(slot + ((ly + dy) * HX + row + dx) * CP + ci)
-                                                         : zero);
+                const u32x4 v = *(const u32x4*)(ring + lane_off + off + m * HX * CP);
+                av[m] = __builtin_bit_cast(bf16x8, ok ? v : zero);
             }
 #pragma unroll
             for (int m = 0; m < MW; ++m)
 #pragma unroll
                 for (int n = 0; n < NT; ++n)
-                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bw[n], acc[m][n], 0, 0, 0);
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[n], av[m], acc[m][n], 0, 0, 0);
         };
         if constexpr (HOLD) {
 #pragma unroll
             for (int kc = 0; kc < KC; ++kc) slice(kc, bhold[kc]);
         } else {
-            bf16x8 bcur[NT], bnext[NT];
+            // weight fragments PD slices ahead in a ring
+            constexpr int PD = 3;
+            bf16x8 bq[PD][NT];
 #pragma unroll
-            for (int n = 0; n < NT; ++n) bcur[n] = b_frag(0, n);
-#pragma unroll 2
+            for (int j = 0; j < PD; ++j)
+#pragma unroll
+                for (int n = 0; n < NT; ++n) bq[j][n] = weight_frag<CIN, CIN, K>(wl, sl, 0, j, n);
+#pragma unroll
             for (int kc = 0; kc < KC; ++kc) {
-                if (kc + 1 < KC) {
+                slice(kc, bq[kc % PD]);
+                if (kc + PD < KC) {
 #pragma unroll
-                    for (int n = 0; n < NT; ++n) bnext[n] = b_frag(kc + 1, n);
+                    for (int n = 0; n < NT; ++n) bq[kc % PD][n] = weight_frag<CIN, CIN, K>(wl, sl, 0, kc + PD, n);
                 }
-                slice(kc, bcur);
-#pragma unroll
-                for (int n = 0; n < NT; ++n) bcur[n] = bnext[n];
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
-        const int z = z0 + oz;
 #pragma unroll
         for (int m = 0; m < MW; ++m) {
-            const int y = y0 + wave * MW + m;
+            const size_t vrow = vb + ((size_t)(z0 + oz) * D + (y0 + wave * MW + m)) * D + x0;
 #pragma unroll
-            for (int n = 0; n < NT; ++n) {
-                const int co = n * 16 + row;
-                const float bv = a.bias ? a.bias[b * a.bias_stride + co] : 0.0f;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int x = x0 + grp * 4 + i;
-                    float v = acc[m][n][i] + bv;
-                    if (a.relu) v = fmaxf(v, 0.0f);
-                    const size_t vox = (((size_t)b * D + z) * D + y) * D + x;
-                    if (co < a.o0)
-                        a.y0[vox * a.o0 + co] = f2bf(v);
-                    else
-                        a.y1[vox * o1 + (co - a.o0)] = f2bf(v);
-                }
-            }
+            for (int n = 0; n < NT; ++n) store_quad<COUT>(a, vrow, row, n, grp, acc[m][n] + bv[n]);
         }
         if (more) store(oz + 3, pf);
         __syncthreads();
@@ -732,42 +853,56 @@ __global__ void k_in_bias(const float* __restrict__ w, const float* __restrict__
     beff[i] = bias[co] + cond[b * 2 + 0] * s1 + cond[b * 2 + 1] * s2;
 }
 
-// y[v][co] = relu(beff[b][co] + sum_t w[co][t][0] * scale * delta[v + off_t]); one thread per voxel.
-template <int COUT>
+// y[v][co] = relu(beff[b][co] + sum_t w[co][t][0] * scale * delta[v + off_t])
+// for the 16 output channels, on the fp32 matrix cores
+// (v_mfma_f32_16x16x4f32): A = scaled weights (16 channels x 4 taps, lane
+// (row, grp) holds channel row, tap 4 s + grp), B = densities (4 taps x the
+// 16 voxels of a line), D = channels x voxels, so a lane ends with channels
+// 4 grp .. +3 of voxel row.  Seven products cover the 27 taps (tap 27 has
+// zero weight).  A block owns TZ x TY lines of 16 voxels with the density
+// halo in LDS; each wave MW lines of one z-plane.
+template <int TZ, int TY>
 __global__ __launch_bounds__(256) void k_in_fwd(const float* __restrict__ delta, const float* __restrict__ w,
-                                                const float* __restrict__ beff, bf16_t* __restrict__ y, int B, int D,
+                                                const float* __restrict__ beff, bf16_t* __restrict__ y, int D,
                                                 float scale) {
-    __shared__ float ws[COUT * 27];
-    for (int i = threadIdx.x; i < COUT * 27; i += 256) ws[i] = w[i * 3];
-    __syncthreads();
-    const size_t nv = (size_t)B * D * D * D;
-    const size_t v = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (v >= nv) return;
-    const int xx = v % D, yy = (v / D) % D, zz = (v / ((size_t)D * D)) % D;
-    const int b = v / ((size_t)D * D * D);
-    float d[27];
-    for (int t = 0; t < 27; ++t) {
-        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-        d[t] = scale * delta[(((size_t)b * D + wrap(zz + dz, D)) * D + wrap(yy + dy, D)) * D + wrap(xx + dx, D)];
+    constexpr int HZ = TZ + 2, HY = TY + 2, HX = 18, MW = TZ * TY / 4;
+    static_assert(TY % MW == 0, "a wave's lines lie in one z-plane");
+    __shared__ float tile[HZ * HY * HX];
+    const int nyt = D / TY;
+    const int x0 = blockIdx.x * 16;
+    const int y0 = (blockIdx.y % nyt) * TY, z0 = (blockIdx.y / nyt) * TZ;
+    const int b = blockIdx.z;
+    const size_t vb = (size_t)b * D * D * D;
+    for (int i = threadIdx.x; i < HZ * HY * HX; i += 256) {
+        const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
+        tile[i] = delta[vb + ((size_t)wrap(z0 + hz - 1, D) * D + wrap(y0 + hy - 1, D)) * D + wrap(x0 + hx - 1, D)];
     }
-    static_assert(COUT % 8 == 0, "16-byte output vectors");
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int row = lane & 15, grp = lane >> 4;
+    float wa[7];
+    int toff[7];
 #pragma unroll
-    for (int c8 = 0; c8 < COUT / 8; ++c8) {
-        u32x4 out;
+    for (int s = 0; s < 7; ++s) {
+        const int t = 4 * s + grp;
+        wa[s] = t < 27 ? w[(row * 27 + t) * 3] * scale : 0.0f;
+        toff[s] = t < 27 ? ((t / 9) * HY + (t / 3) % 3) * HX + t % 3 : 0;
+    }
+    const floatx4 bias = *(const floatx4*)(beff + b * 16 + grp * 4);
+    __syncthreads();
+    const int mt0 = wave * MW, lz = mt0 / TY, ly0 = mt0 % TY;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            unsigned pair = 0;
+    for (int m = 0; m < MW; ++m) {
+        const float* lt = tile + (lz * HY + ly0 + m) * HX + row;
+        floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int co = c8 * 8 + 2 * k + h;
-                float acc = beff[b * COUT + co];
-#pragma unroll
-                for (int t = 0; t < 27; ++t) acc = fmaf(ws[co * 27 + t], d[t], acc);
-                pair |= (unsigned)f2bf(fmaxf(acc, 0.0f)) << (16 * h);
-            }
-            out[k] = pair;
-        }
-        *(u32x4*)(y + v * COUT + c8 * 8) = out;
+        for (int s = 0; s < 7; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(wa[s], lt[toff[s]], acc, 0, 0, 0);
+        acc += bias;
+        u32x2 q;
+        q[0] = f2bf2(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f));
+        q[1] = f2bf2(fmaxf(acc[2], 0.0f), fmaxf(acc[3], 0.0f));
+        const size_t vrow = vb + ((size_t)(z0 + lz) * D + (y0 + ly0 + m)) * D + x0;
+        *(u32x2*)(y + vrow * 16 + row * 16 + grp * 4) = q;
     }
 }
 
@@ -963,10 +1098,10 @@ __global__ void k_adamw(float* __restrict__ p, const float* __restrict__ g, floa
 // ------------------------------------------------------------------ dispatch
 
 using ConvFn = void (*)(const ConvArgs&, hipStream_t);
-template <int CIN, int COUT, int TZ, int TY, int WAVES = 4>
+template <int CIN, int COUT, int TZ, int TY, int CH = CIN>
 void launch_conv(const ConvArgs& a, hipStream_t st) {
     const dim3 grid(a.D / 16, (a.D / TZ) * (a.D / TY), a.B);
-    k_conv3_mfma<CIN, COUT, TZ, TY, WAVES><<<grid, 64 * WAVES, 0, st>>>(a);
+    k_conv3_mfma<CIN, COUT, TZ, TY, CH><<<grid, 256, 0, st>>>(a);
 }
 template <int CIN, int COUT, int TY, int NZ>
 void launch_slide(const ConvArgs& a, hipStream_t st) {
@@ -977,15 +1112,17 @@ void launch_slide(const ConvArgs& a, hipStream_t st) {
 // Tiled form: 4 x 4 x 16 (cin <= 32) or 4 x 2 x 16 (cin = 64) voxels per
 // block for D >= 64, 2 x 2 x 16 below (enough blocks to fill the chip).
 // Sliding form: 8 (cin <= 32) or 4 lines of 16 voxels over 16 planes
-// (D >= 128) or 8 planes (D = 64).
+// (D >= 128) or 8 planes (D = 64).  Chunked tile: 16 input channels at a
+// time, 4 x 8 x 16 voxels per block (2 x 8 x 16 for 64 output channels).
 struct ConvShape {
     int cin, cout;
-    ConvFn tile_big, tile_small, slide128, slide64;
+    ConvFn tile_big, tile_small, slide128, slide64, chunked;
 };
 
 #define CM21_CONV(CI, CO)                                                                             \
     {CI, CO, launch_conv<CI, CO, 4, (CI >= 64 ? 2 : 4)>, launch_conv<CI, CO, 2, 2>,                    \
-     launch_slide<CI, CO, (CI >= 64 ? 4 : 8), 16>, launch_slide<CI, CO, (CI >= 64 ? 4 : 8), 8>}
+     launch_slide<CI, CO, (CI >= 64 ? 4 : 8), 16>, launch_slide<CI, CO, (CI >= 64 ? 4 : 8), 8>,       \
+     launch_conv<CI, CO, (CO >= 64 ? 2 : 4), 8, 16>}
 const ConvShape kConvShapes[] = {
     CM21_CONV(16, 16), CM21_CONV(16, 32), CM21_CONV(32, 16), CM21_CONV(32, 32),
     CM21_CONV(32, 64), CM21_CONV(64, 32), CM21_CONV(64, 64),
@@ -1066,13 +1203,17 @@ extern "C" int cm21_unet_conv3_fwd(const cm21_unet_conv* c, void* stream) {
     a.y0 = (bf16_t*)c->y0;
     a.y1 = (bf16_t*)c->y1;
     hipStream_t st = (hipStream_t)stream;
-    CM21_REQUIRE(c->algo >= 0 && c->algo <= 2, "cm21_unet_conv3_fwd: algo=%d", c->algo);
-    CM21_REQUIRE(c->algo < 2 || c->D >= 64, "cm21_unet_conv3_fwd: algo %d needs D >= 64", c->algo);
+    CM21_REQUIRE(c->algo >= 0 && c->algo <= 3, "cm21_unet_conv3_fwd: algo=%d", c->algo);
+    CM21_REQUIRE(c->algo != 2 || c->D >= 64, "cm21_unet_conv3_fwd: algo 2 needs D >= 64");
     // automatic choice from per-layer timings on the MI355X: the sliding
-    // window wins for 16 -> 16 channels at D >= 128, one tile per block
-    // elsewhere
-    const int algo = c->algo ? c->algo : ((c->cin == 16 && c->cout == 16 && c->D >= 128) ? 2 : 1);
-    ConvFn fn = c->D < 64 ? s->tile_small : (algo == 1 ? s->tile_big : (c->D >= 128 ? s->slide128 : s->slide64));
+    // window wins for 16 -> 16 channels at D >= 128, the tile of 16 input
+    // channels at a time elsewhere (up to 3x faster than the whole-channel
+    // tile for 64 input channels, whose weights would otherwise be fetched
+    // per pair of lines)
+    const int algo = c->algo ? c->algo : ((c->cin == 16 && c->cout == 16 && c->D >= 128) ? 2 : 3);
+    ConvFn fn = algo == 3 ? s->chunked
+                          : (c->D < 64 ? s->tile_small
+                                       : (algo == 1 ? s->tile_big : (c->D >= 128 ? s->slide128 : s->slide64)));
     fn(a, st);
     CM21_LAUNCH_CHECK("k_conv3_mfma");
     return CM21_OK;
@@ -1215,8 +1356,7 @@ extern "C" int cm21_unet_in_fwd(const float* delta, const float* cond, const flo
     hipStream_t st = (hipStream_t)stream;
     k_in_bias<<<(B * 16 + 63) / 64, 64, 0, st>>>(w, bias, cond, beff, B, 16);
     CM21_LAUNCH_CHECK("k_in_bias");
-    const size_t nv = (size_t)B * D * D * D;
-    k_in_fwd<16><<<(unsigned)((nv + 255) / 256), 256, 0, st>>>(delta, w, beff, (bf16_t*)y, B, D, delta_scale);
+    k_in_fwd<4, 8><<<dim3(D / 16, (D / 4) * (D / 8), B), 256, 0, st>>>(delta, w, beff, (bf16_t*)y, D, delta_scale);
     CM21_LAUNCH_CHECK("k_in_fwd");
     return CM21_OK;
 }

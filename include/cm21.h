@@ -289,7 +289,8 @@ typedef struct cm21_unet_conv {
     void* y0;            /* bf16 (B,D,D,D,o0) */
     void* y1;            /* bf16 (B,D,D,D,cout-o0) or NULL */
     int algo;            /* 0 = automatic, 1 = one halo tile per block,
-                            2 = sliding window along z (D >= 64) */
+                            2 = sliding window along z (D >= 64),
+                            3 = halo tile of 16 input channels at a time */
 } cm21_unet_conv;
 /* y = conv(x) on the matrix cores; (cin, cout) in {(16,16), (16,32), (32,16),
  * (32,32), (32,64), (64,32), (64,64)}, D a multiple of 16.  The input

@@ -65,9 +65,10 @@ def _conv_call(lib, B, D, cin, cout, x0, w, y0, x1=None, c0=None, mask=None, bia
     lib.call("cm21_unet_conv3_fwd", ctypes.byref(c), lib.stream_handle())
 
 
-# D < 64: one 2 x 2 x 16 tile per block; D >= 64: both the 4 x 4 x 16 (or
+# D < 64: one 2 x 2 x 16 tile per block; D >= 64: the 4 x 4 x 16 (or
 # 4 x 2 x 16) tile per block (algo 1) and the sliding window over 8 (D = 64)
-# or 16 planes (D >= 128, algo 2)
+# or 16 planes (D >= 128, algo 2); every D: the tile of 16 input channels at
+# a time (algo 3)
 SHAPES = [(16, 16, 16, False), (16, 32, 32, False), (32, 16, 64, True), (32, 32, 16, False), (32, 64, 16, False),
           (64, 32, 32, True), (64, 64, 16, False), (64, 64, 64, False), (16, 32, 64, False), (32, 32, 64, False),
           (64, 32, 64, True), (16, 16, 128, False), (32, 16, 128, True)]
@@ -90,7 +91,7 @@ def test_conv3_fwd(gpu, cin, cout, D, split):
         bb = _d(bias[b]) if split else _d(bias)
         ref.append(unet.conv3(_d(x[b:b + 1]), _d(w), bb)[0])
     ref = torch.stack(ref)
-    for algo in ((1, 2) if D >= 64 else (0,)):
+    for algo in ((1, 2, 3) if D >= 64 else (0, 3)):
         print(f"conv fwd cin={cin} cout={cout} D={D} split={split} algo={algo}", flush=True)
         y0.fill_(float("nan"))
         _conv_call(lib, B, D, cin, cout, x0, w, y0, x1=x1, c0=c0, bias=bias, y1=y1, o0=o0, per_sample=split,
@@ -183,9 +184,10 @@ def test_pool_and_up(gpu):
         _close_f32(db, br.grad, f"up db {cin}->{cout}")
 
 
-def test_input_layer(gpu):
+@pytest.mark.parametrize("D", [32, 64])
+def test_input_layer(gpu, D):
     lib = gpu
-    B, D = 2, 32
+    B = 2
     delta = _f((B, D, D, D), 0.6, 31)
     cond = torch.tensor([[0.3, -0.7], [-1.0, 0.9]], dtype=torch.float32, device="cuda")
     w = _f((16, 27, 3), np.sqrt(2.0 / 81), 32)
