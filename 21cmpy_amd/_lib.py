@@ -85,6 +85,7 @@ class UnetConv(ctypes.Structure):
         ("x0", ctypes.c_void_p), ("x1", ctypes.c_void_p), ("mask", ctypes.c_void_p), ("w", ctypes.c_void_p),
         ("bias", ctypes.POINTER(ctypes.c_float)), ("bias_per_sample", ctypes.c_int), ("relu", ctypes.c_int),
         ("y0", ctypes.c_void_p), ("y1", ctypes.c_void_p), ("algo", ctypes.c_int),
+        ("head", ctypes.POINTER(ctypes.c_float)), ("prob", ctypes.POINTER(ctypes.c_float)),
     ]
 
 

@@ -88,6 +88,8 @@ struct ConvArgs {
     int relu;
     bf16_t* y0;
     bf16_t* y1;
+    const float* head;  // fused 1^3 head (cout = 16): w[16], b; or NULL
+    float* prob;        // its sigmoid output (B,D,D,D) fp32
 };
 
 // The MFMA takes the weights as its A operand and the activations as its B
@@ -160,6 +162,40 @@ __device__ __forceinline__ void store_quad(const ConvArgs& a, size_t vrow, int r
         bf16_t* p = a.y1 + vrow * o1;
         *(u32x2*)(p + row * o1 + n * 16 - a.o0 + grp * 4) = q;
     }
+}
+
+// The head's logit b + sum_c w[c] x[c] over 16 channels is summed in a fixed
+// order shared by the head kernel and the fused form: four partials of four
+// consecutive channels, p_g = fma chain over channels 4g .. 4g+3, then
+// ((p0 + p1) + (p2 + p3)) + b.
+__device__ __forceinline__ float head_partial(const float* w4, const float* x4) {
+    float p = w4[0] * x4[0];
+    p = fmaf(w4[1], x4[1], p);
+    p = fmaf(w4[2], x4[2], p);
+    return fmaf(w4[3], x4[3], p);
+}
+// sigmoid on the transcendental unit (v_exp_f32, v_rcp_f32: about 1 ulp each)
+__device__ __forceinline__ float head_sigmoid(float lg) { return __builtin_amdgcn_rcpf(1.0f + __expf(-lg)); }
+
+// Fused head (inference, cout = 16): lane group grp holds channels 4 grp ..
+// +3 of voxel vrow + row; rounded to bf16 as the stored activation would be,
+// they give partial p_grp, and two butterflies over the lane groups form the
+// logit in the shared order (fp addition is commutative, so every lane of the
+// row holds the same value).
+__device__ __forceinline__ void head_quad(const ConvArgs& a, size_t vrow, int row, int grp, floatx4 v) {
+    const floatx4 w = *(const floatx4*)(a.head + grp * 4);  // the lane group's 4 head weights
+    float xr[4], w4[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float r = a.relu ? fmaxf(v[i], 0.0f) : v[i];
+        xr[i] = __uint_as_float((unsigned)f2bf(r) << 16);
+        w4[i] = w[i];
+    }
+    const float p = head_partial(w4, xr);
+    const float s1 = p + __shfl_xor(p, 16, 64);     // p0 + p1 (groups 0, 1), p2 + p3 (groups 2, 3)
+    const float s2 = s1 + __shfl_xor(s1, 32, 64);   // (p0 + p1) + (p2 + p3)
+    const float prob = head_sigmoid(s2 + a.head[16]);
+    if (grp == 0) a.prob[vrow + row] = prob;
 }
 
 // bias of output channels 16 n + 4 grp .. +3
@@ -310,6 +346,16 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
     }
     floatx4 bv[NT];
     load_bias<NT>(a, b, grp, bv);
+    if constexpr (COUT == 16) {
+        if (a.head) {
+#pragma unroll
+            for (int m = 0; m < MW; ++m) {
+                const size_t vrow = vb + ((size_t)(z0 + lz) * D + (y0 + ly0 + m)) * D + x0;
+                head_quad(a, vrow, row, grp, acc[m][0] + bv[0]);
+            }
+            return;
+        }
+    }
 #pragma unroll
     for (int m = 0; m < MW; ++m) {
         const size_t vrow = vb + ((size_t)(z0 + lz) * D + (y0 + ly0 + m)) * D + x0;
@@ -325,7 +371,7 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
 // most 14 weight fragments (K slices x N tiles) keep every B fragment in
 // registers for the whole walk; the others stream them per K slice,
 // double-buffered.
-template <int CIN, int COUT, int TY, int NZ>
+template <int CIN, int COUT, int TY, int NZ, bool HEAD = false>
 __global__ __launch_bounds__(256) void k_conv3_slide(ConvArgs a) {
     constexpr int HY = TY + 2, HX = 18, CP = CIN >= 32 ? CIN + 8 : CIN;
     constexpr int PLANE = HY * HX * CP;               // bf16 per ring slot
@@ -459,11 +505,22 @@ __global__ __launch_bounds__(256) void k_conv3_slide(ConvArgs a) {
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
+        // the fused head is its own instance: its epilogue registers would
+        // cost the plain instance a wave per SIMD
+        if constexpr (HEAD) {
+            static_assert(COUT == 16, "the fused head reads 16 channels");
 #pragma unroll
-        for (int m = 0; m < MW; ++m) {
-            const size_t vrow = vb + ((size_t)(z0 + oz) * D + (y0 + wave * MW + m)) * D + x0;
+            for (int m = 0; m < MW; ++m) {
+                const size_t vrow = vb + ((size_t)(z0 + oz) * D + (y0 + wave * MW + m)) * D + x0;
+                head_quad(a, vrow, row, grp, acc[m][0] + bv[0]);
+            }
+        } else {
 #pragma unroll
-            for (int n = 0; n < NT; ++n) store_quad<COUT>(a, vrow, row, n, grp, acc[m][n] + bv[n]);
+            for (int m = 0; m < MW; ++m) {
+                const size_t vrow = vb + ((size_t)(z0 + oz) * D + (y0 + wave * MW + m)) * D + x0;
+#pragma unroll
+                for (int n = 0; n < NT; ++n) store_quad<COUT>(a, vrow, row, n, grp, acc[m][n] + bv[n]);
+            }
         }
         if (more) store(oz + 3, pf);
         __syncthreads();
@@ -649,21 +706,31 @@ __global__ void k_channel_final(const float* __restrict__ partial, int B, int nb
 
 // ------------------------------------------------------------------ pooling
 
-__global__ void k_pool_fwd(const bf16_t* __restrict__ x, bf16_t* __restrict__ y, int B, int D, int C) {
-    const int Dh = D / 2;
-    const size_t n = (size_t)B * Dh * Dh * Dh * C;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = i % C;
-        size_t v = i / C;
-        const int x0 = v % Dh, y0 = (v / Dh) % Dh, z0 = (v / ((size_t)Dh * Dh)) % Dh;
-        const int b = v / ((size_t)Dh * Dh * Dh);
-        float s = 0.0f;
-        for (int q = 0; q < 8; ++q) {
-            const size_t u = (((size_t)b * D + 2 * z0 + (q >> 2)) * D + 2 * y0 + ((q >> 1) & 1)) * D + 2 * x0 + (q & 1);
-            s += bf2f(x[u * C + c]);
+// One thread per (output voxel, 8 channels): eight 16-byte loads, the mean
+// in fp32 (the eight inputs in z, y, x order), one 16-byte store.  Grid:
+// (x and channel chunks of a row, output y, output z * B).
+__global__ __launch_bounds__(64) void k_pool_fwd(const bf16_t* __restrict__ x, bf16_t* __restrict__ y, int D, int C) {
+    const int Dh = D / 2, C8 = C / 8;
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= Dh * C8) return;
+    const int ox = i / C8, cc = (i - ox * C8) * 8;
+    const int oy = blockIdx.y, oz = blockIdx.z % Dh, b = blockIdx.z / Dh;
+    const size_t in0 = ((((size_t)b * D + 2 * oz) * D + 2 * oy) * D + 2 * ox) * C + cc;
+    const size_t dy = (size_t)D * C, dz = (size_t)D * D * C;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const u32x4 v = *(const u32x4*)(x + in0 + (q >> 2) * dz + ((q >> 1) & 1) * dy + (q & 1) * C);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            s[2 * k] += __uint_as_float(v[k] << 16);
+            s[2 * k + 1] += __uint_as_float(v[k] & 0xffff0000u);
         }
-        y[i] = f2bf(s * 0.125f);
     }
+    u32x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = f2bf2(s[2 * k] * 0.125f, s[2 * k + 1] * 0.125f);
+    *(u32x4*)(y + ((((size_t)b * Dh + oz) * Dh + oy) * Dh + ox) * C + cc) = o;
 }
 
 __global__ void k_pool_bwd(const bf16_t* __restrict__ dy, bf16_t* __restrict__ dx, int B, int D, int C, int add) {
@@ -693,36 +760,41 @@ template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void k_up_fwd_mfma(const bf16_t* __restrict__ x, const bf16_t* __restrict__ wn,
                                                      const float* __restrict__ bias, bf16_t* __restrict__ y, int B,
                                                      int D) {
+    // weights are the A operand (N = s * COUT + co), voxels the B operand:
+    // lane (row, grp) ends with outputs n = 16 j + 4 grp .. +3 of input voxel
+    // v0 + row, i.e. 4 consecutive channels of one output voxel (16 j lies
+    // inside one sub-position s because COUT is a multiple of 16)
     constexpr int KC = CIN / 32, NT = 8 * COUT / 16;
     static_assert(CIN % 32 == 0 && COUT % 16 == 0, "channel counts");
     const int lane = threadIdx.x & 63, row = lane & 15, grp = lane >> 4;
     const size_t nseg = (size_t)B * D * D * D / 16;
-    const size_t seg = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const size_t seg = (size_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (seg >= nseg) return;  // whole wave
     const size_t v0 = seg * 16;
     bf16x8 av[KC];
 #pragma unroll
     for (int k = 0; k < KC; ++k) av[k] = __builtin_bit_cast(bf16x8, *(const u32x4*)(x + (v0 + row) * CIN + k * 32 + grp * 8));
-    const int D2 = 2 * D;
+    const size_t D2 = 2 * (size_t)D;
+    // output voxel of this lane's input voxel at sub-position 0
+    const size_t v = v0 + row;
+    const size_t xx = v % D, yy = (v / D) % D, zz = (v / ((size_t)D * D)) % D, b = v / ((size_t)D * D * D);
+    const size_t u00 = ((b * D2 + 2 * zz) * D2 + 2 * yy) * D2 + 2 * xx;
 #pragma unroll 2
     for (int j = 0; j < NT; ++j) {
         floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
             const bf16x8 bw = __builtin_bit_cast(bf16x8, *(const u32x4*)(wn + (size_t)(j * 16 + row) * CIN + k * 32 + grp * 8));
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[k], bw, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw, av[k], acc, 0, 0, 0);
         }
-        const int n = j * 16 + row;
-        const int s = n / COUT, co = n % COUT;
-        const float bv = bias[co];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const size_t v = v0 + grp * 4 + i;
-            const int xx = v % D, yy = (v / D) % D, zz = (v / ((size_t)D * D)) % D;
-            const int b = v / ((size_t)D * D * D);
-            const size_t u = (((size_t)b * D2 + 2 * zz + (s >> 2)) * D2 + 2 * yy + ((s >> 1) & 1)) * D2 + 2 * xx + (s & 1);
-            y[u * COUT + co] = f2bf(acc[i] + bv);
-        }
+        const int sub = (j * 16) / COUT, co = (j * 16) % COUT + grp * 4;
+        const floatx4 bv = *(const floatx4*)(bias + co);
+        acc += bv;
+        const size_t u = u00 + ((sub >> 2) * D2 + ((sub >> 1) & 1)) * D2 + (sub & 1);
+        u32x2 q;
+        q[0] = f2bf2(acc[0], acc[1]);
+        q[1] = f2bf2(acc[2], acc[3]);
+        *(u32x2*)(y + u * COUT + co) = q;
     }
 }
 
@@ -1022,10 +1094,11 @@ __global__ __launch_bounds__(256) void k_head(const bf16_t* __restrict__ x, cons
                 xv[q * 8 + 2 * k + 1] = __uint_as_float(r[k] & 0xffff0000u);
             }
         }
-        float lg = b0;
-#pragma unroll
-        for (int c = 0; c < C; ++c) lg = fmaf(wv[c], xv[c], lg);
-        const float p = 1.0f / (1.0f + expf(-lg));
+        static_assert(C == 16, "the shared summation order is defined for 16 channels");
+        const float p0 = head_partial(wv, xv), p1 = head_partial(wv + 4, xv + 4);
+        const float p2 = head_partial(wv + 8, xv + 8), p3 = head_partial(wv + 12, xv + 12);
+        const float lg = ((p0 + p1) + (p2 + p3)) + b0;
+        const float p = head_sigmoid(lg);
         if (prob) prob[v] = p;
         if (target) {
             const float tv = target[v];
@@ -1103,10 +1176,10 @@ void launch_conv(const ConvArgs& a, hipStream_t st) {
     const dim3 grid(a.D / 16, (a.D / TZ) * (a.D / TY), a.B);
     k_conv3_mfma<CIN, COUT, TZ, TY, CH><<<grid, 256, 0, st>>>(a);
 }
-template <int CIN, int COUT, int TY, int NZ>
+template <int CIN, int COUT, int TY, int NZ, bool HEAD = false>
 void launch_slide(const ConvArgs& a, hipStream_t st) {
     const dim3 grid(a.D / 16, (a.D / TY) * (a.D / NZ), a.B);
-    k_conv3_slide<CIN, COUT, TY, NZ><<<grid, 256, 0, st>>>(a);
+    k_conv3_slide<CIN, COUT, TY, NZ, HEAD><<<grid, 256, 0, st>>>(a);
 }
 
 // Tiled form: 4 x 4 x 16 (cin <= 32) or 4 x 2 x 16 (cin = 64) voxels per
@@ -1117,12 +1190,14 @@ void launch_slide(const ConvArgs& a, hipStream_t st) {
 struct ConvShape {
     int cin, cout;
     ConvFn tile_big, tile_small, slide128, slide64, chunked;
+    ConvFn slide128_head, slide64_head;  // sliding form with the fused head (cout = 16)
 };
 
 #define CM21_CONV(CI, CO)                                                                             \
     {CI, CO, launch_conv<CI, CO, 4, (CI >= 64 ? 2 : 4)>, launch_conv<CI, CO, 2, 2>,                    \
      launch_slide<CI, CO, (CI >= 64 ? 4 : 8), 16>, launch_slide<CI, CO, (CI >= 64 ? 4 : 8), 8>,       \
-     launch_conv<CI, CO, (CO >= 64 ? 2 : 4), 8, 16>}
+     launch_conv<CI, CO, (CO >= 64 ? 2 : 4), 8, 16>,                                                  \
+     launch_slide<CI, CO, (CI >= 64 ? 4 : 8), 16, CO == 16>, launch_slide<CI, CO, (CI >= 64 ? 4 : 8), 8, CO == 16>}
 const ConvShape kConvShapes[] = {
     CM21_CONV(16, 16), CM21_CONV(16, 32), CM21_CONV(32, 16), CM21_CONV(32, 32),
     CM21_CONV(32, 64), CM21_CONV(64, 32), CM21_CONV(64, 64),
@@ -1184,7 +1259,9 @@ extern "C" int cm21_unet_conv3_fwd(const cm21_unet_conv* c, void* stream) {
     CM21_REQUIRE(c->B >= 1 && valid_unet_dim(c->D), "cm21_unet_conv3_fwd: B=%d D=%d", c->B, c->D);
     CM21_REQUIRE(c->c0 > 0 && c->c0 <= c->cin && c->c0 % 8 == 0, "cm21_unet_conv3_fwd: c0=%d", c->c0);
     CM21_REQUIRE(c->o0 > 0 && c->o0 <= c->cout && c->o0 % 16 == 0, "cm21_unet_conv3_fwd: o0=%d", c->o0);
-    CM21_REQUIRE(c->x0 && c->w && c->y0, "cm21_unet_conv3_fwd: NULL x0, w or y0");
+    CM21_REQUIRE(c->x0 && c->w && (c->y0 || c->head), "cm21_unet_conv3_fwd: NULL x0, w or y0");
+    CM21_REQUIRE(!c->head || (c->cout == 16 && c->o0 == 16 && c->prob),
+                 "cm21_unet_conv3_fwd: the fused head needs cout = 16, o0 = 16 and prob");
     CM21_REQUIRE(c->c0 == c->cin || c->x1, "cm21_unet_conv3_fwd: x1 is NULL with c0 < cin");
     CM21_REQUIRE(c->o0 == c->cout || c->y1, "cm21_unet_conv3_fwd: y1 is NULL with o0 < cout");
     CM21_REQUIRE(!c->mask || c->c0 == c->cin, "cm21_unet_conv3_fwd: mask needs a single input");
@@ -1202,6 +1279,8 @@ extern "C" int cm21_unet_conv3_fwd(const cm21_unet_conv* c, void* stream) {
     a.relu = c->relu;
     a.y0 = (bf16_t*)c->y0;
     a.y1 = (bf16_t*)c->y1;
+    a.head = c->head;
+    a.prob = c->prob;
     hipStream_t st = (hipStream_t)stream;
     CM21_REQUIRE(c->algo >= 0 && c->algo <= 3, "cm21_unet_conv3_fwd: algo=%d", c->algo);
     CM21_REQUIRE(c->algo != 2 || c->D >= 64, "cm21_unet_conv3_fwd: algo 2 needs D >= 64");
@@ -1210,10 +1289,14 @@ extern "C" int cm21_unet_conv3_fwd(const cm21_unet_conv* c, void* stream) {
     // channels at a time elsewhere (up to 3x faster than the whole-channel
     // tile for 64 input channels, whose weights would otherwise be fetched
     // per pair of lines)
-    const int algo = c->algo ? c->algo : ((c->cin == 16 && c->cout == 16 && c->D >= 128) ? 2 : 3);
+    // (the fused head goes on the chunked tile, whose occupancy is set by
+    // its LDS, so the head's epilogue registers cost nothing)
+    const int algo =
+        c->algo ? c->algo : ((c->cin == 16 && c->cout == 16 && c->D >= 128 && !c->head) ? 2 : 3);
     ConvFn fn = algo == 3 ? s->chunked
                           : (c->D < 64 ? s->tile_small
                                        : (algo == 1 ? s->tile_big : (c->D >= 128 ? s->slide128 : s->slide64)));
+    if (c->head && algo == 2) fn = c->D >= 128 ? s->slide128_head : s->slide64_head;
     fn(a, st);
     CM21_LAUNCH_CHECK("k_conv3_mfma");
     return CM21_OK;
@@ -1263,9 +1346,11 @@ extern "C" int cm21_unet_conv3_wgrad(const cm21_unet_wgrad* w, void* stream) {
 }
 
 extern "C" int cm21_unet_pool_fwd(const void* x, void* y, int B, int D, int C, void* stream) {
-    CM21_REQUIRE(x && y && B >= 1 && D >= 2 && D % 2 == 0 && C >= 1, "cm21_unet_pool_fwd: bad arguments");
-    const size_t n = (size_t)B * (D / 2) * (D / 2) * (D / 2) * C;
-    k_pool_fwd<<<grid_for(n), kThreads, 0, (hipStream_t)stream>>>((const bf16_t*)x, (bf16_t*)y, B, D, C);
+    CM21_REQUIRE(x && y && B >= 1 && D >= 2 && D % 2 == 0 && C >= 8 && C % 8 == 0 && (size_t)B * (D / 2) < 65536,
+                 "cm21_unet_pool_fwd: bad arguments");
+    const int Dh = D / 2;
+    k_pool_fwd<<<dim3((Dh * (C / 8) + 63) / 64, Dh, B * Dh), 64, 0, (hipStream_t)stream>>>((const bf16_t*)x,
+                                                                                          (bf16_t*)y, D, C);
     CM21_LAUNCH_CHECK("k_pool_fwd");
     return CM21_OK;
 }

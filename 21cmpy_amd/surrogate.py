@@ -194,10 +194,10 @@ class UNet3D:
 
     # -------------------------------------------------------------- layers
     def _conv(self, x0, w, y0, cin, cout, D, x1=None, mask=None, bias=None, relu=True, y1=None, c0=None, o0=None,
-              bias_per_sample=False):
+              bias_per_sample=False, head=None, prob=None):
         c = _lib.UnetConv(B=self.B, D=D, cin=cin, cout=cout, c0=c0 or cin, o0=o0 or cout, x0=_vp(x0), x1=_vp(x1),
                           mask=_vp(mask), w=_vp(w), bias=_fp(bias), bias_per_sample=1 if bias_per_sample else 0,
-                          relu=1 if relu else 0, y0=_vp(y0), y1=_vp(y1))
+                          relu=1 if relu else 0, y0=_vp(y0), y1=_vp(y1), head=_fp(head), prob=_fp(prob))
         _lib.call("cm21_unet_conv3_fwd", ctypes.byref(c), self._st())
 
     def _wgrad(self, x0, g, mask, name, cin, cout, D, x1=None, c0=None):
@@ -207,9 +207,12 @@ class UNet3D:
         _lib.call("cm21_unet_conv3_wgrad", ctypes.byref(w), self._st())
 
     # -------------------------------------------------------------- passes
-    def forward(self, delta, cond):
+    def forward(self, delta, cond, keep=True):
         """delta: (B,D,D,D) fp32 CUDA tensor; cond: (B,2) normalised
-        conditioning.  Returns the probability tensor (B,D,D,D) (a buffer)."""
+        conditioning.  Returns the probability tensor (B,D,D,D) (a buffer).
+        keep=False (inference) fuses the head into the last convolution,
+        whose activations are then not stored, so backward() needs a
+        forward with keep=True; both give the same probabilities."""
         if tuple(delta.shape) != (self.B, self.D, self.D, self.D) or delta.dtype != torch.float32:
             raise ValueError(f"delta must be float32 {(self.B, self.D, self.D, self.D)}")
         if not delta.is_contiguous():
@@ -235,14 +238,21 @@ class UNet3D:
         _lib.call("cm21_unet_up_fwd", _vp(a["d2"]), _vp(self.wb["up1"]), _fp(p["up1.b"]), _vp(a["u1"]), self.B, h, 32,
                   16, st)
         self._conv(a["e1"], self.wb["d1a"], a["d1a"], 32, 16, D, x1=a["u1"], c0=16, bias=p["d1a.b"])
-        self._conv(a["d1a"], self.wb["d1b"], a["d1"], 16, 16, D, bias=p["d1b.b"])
-        _lib.call("cm21_unet_head", _vp(a["d1"]), _fp(p["head.w"]), _fp(p["head.b"]), None, _fp(self.prob), None,
-                  None, None, self.B, D, None, 0, st)
+        self._kept = bool(keep)
+        if keep:
+            self._conv(a["d1a"], self.wb["d1b"], a["d1"], 16, 16, D, bias=p["d1b.b"])
+            _lib.call("cm21_unet_head", _vp(a["d1"]), _fp(p["head.w"]), _fp(p["head.b"]), None, _fp(self.prob), None,
+                      None, None, self.B, D, None, 0, st)
+        else:
+            # head.w (16) and head.b (1) are adjacent in the flat parameter buffer
+            self._conv(a["d1a"], self.wb["d1b"], None, 16, 16, D, bias=p["d1b.b"], head=p["head.w"], prob=self.prob)
         return self.prob
 
     def backward(self, target):
         """Loss and gradients of the last forward against target (B,D,D,D)
         fp32; fills self.grad.  Returns the loss as a device tensor."""
+        if not getattr(self, "_kept", False):
+            raise RuntimeError("backward() needs the activations of a forward(..., keep=True)")
         a, ga, p, D = self.a, self.ga, self.p, self.D
         h, q = D // 2, D // 4
         st = self._st()
@@ -378,7 +388,7 @@ def evaluate(model, boxes, idx=None):
         if len(sel) < model.B:
             sel = sel + [sel[-1]] * (model.B - len(sel))
         delta = boxes.delta[sel].contiguous()
-        prob = model.forward(delta, boxes.cond[sel])
+        prob = model.forward(delta, boxes.cond[sel], keep=False)
         for j, i in enumerate(idx[i0:i0 + model.B]):
             out.append(box_metrics(prob[j], boxes.target[i], boxes.delta[i], boxes.L))
     return out
@@ -519,7 +529,7 @@ class SurrogatePipeline:
             self.fallbacks += 1
             return self._plain(delta, zeta, smin2), "plain"
         cond = [normalise_cond(z, s) for z, s in zip(zeta, smin2)]
-        prob = self.model.forward(delta.contiguous(), cond).clone()
+        prob = self.model.forward(delta.contiguous(), cond, keep=False).clone()
         check = [j for j in range(B) if (self.seen + j) % self.validate_every == 0]
         self.seen += B
         if check:

@@ -221,14 +221,14 @@ def surrogate_bench(args, torch, plain_boxes_s=None):
     delta = boxes.delta[:B].contiguous()
     cond = boxes.cond[:B]
     for _ in range(2):
-        model.forward(delta, cond)
+        model.forward(delta, cond, keep=False)
     steps = max(args.steps, 1)
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     start.record()
     for _ in range(steps):
-        model.forward(delta, cond)
+        model.forward(delta, cond, keep=False)
     end.record()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0

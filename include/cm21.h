@@ -291,6 +291,10 @@ typedef struct cm21_unet_conv {
     int algo;            /* 0 = automatic, 1 = one halo tile per block,
                             2 = sliding window along z (D >= 64),
                             3 = halo tile of 16 input channels at a time */
+    const float* head;   /* fp32 w[16], b of a 1^3 head fused into this conv
+                            (cout = 16, inference): prob receives
+                            sigmoid(b + w . y) and y0 is not written; or NULL */
+    float* prob;         /* fp32 (B,D,D,D) when head is set */
 } cm21_unet_conv;
 /* y = conv(x) on the matrix cores; (cin, cout) in {(16,16), (16,32), (32,16),
  * (32,32), (32,64), (64,32), (64,64)}, D a multiple of 16.  The input
@@ -312,8 +316,9 @@ typedef struct cm21_unet_wgrad {
 size_t cm21_unet_wgrad_workspace_bytes(int B, int D, int cin, int cout);
 int cm21_unet_conv3_wgrad(const cm21_unet_wgrad* w, void* stream);
 
-/* 2x2x2 mean pooling: x (B,D,D,D,C) -> y (B,D/2,D/2,D/2,C); backward
- * dx (+)= dy/8 broadcast (accumulate = 1 adds to dx). */
+/* 2x2x2 mean pooling: x (B,D,D,D,C) -> y (B,D/2,D/2,D/2,C), C a multiple
+ * of 8 for the forward; backward dx (+)= dy/8 broadcast (accumulate = 1 adds
+ * to dx). */
 int cm21_unet_pool_fwd(const void* x, void* y, int B, int D, int C, void* stream);
 int cm21_unet_pool_bwd(const void* dy, void* dx, int B, int D, int C, int accumulate, void* stream);
 

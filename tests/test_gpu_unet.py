@@ -359,3 +359,43 @@ def test_pipeline_validates_and_falls_back(gpu):
         for j in range(4):
             assert torch.equal(fields[j], boxes.target[j]), (rep, j)
     assert bad.failures == 2 and bad.disabled and bad.fallbacks == 3
+
+
+@pytest.mark.parametrize("D", [64, 128])
+def test_fused_head_matches_unfused(gpu, D):
+    """Inference fuses the head into the last convolution; the probabilities
+    equal those of the separate head kernel bit for bit (both sum the logit
+    in the same order), every conv form that carries the head agrees, and
+    backward() refuses to run on the activations the fused pass did not
+    store."""
+    sur = pkg("surrogate")
+    B = 1
+    P = sur.init_params(3)
+    rng = np.random.default_rng(4)
+    for k in P:
+        if k.endswith(".b"):
+            P[k] = rng.standard_normal(P[k].shape) * 0.05
+    model = sur.UNet3D(D=D, batch=B, params=P)
+    delta, cond, _ = _net_inputs(D, B, 12)
+    d = torch.from_numpy(delta).cuda()
+    p_sep = model.forward(d, cond).clone()
+    p_fused = model.forward(d, cond, keep=False).clone()
+    torch.cuda.synchronize()
+    print(f"D={D}: max |diff| {float((p_sep - p_fused).abs().max()):.3g}", flush=True)
+    assert torch.equal(p_sep, p_fused)
+    with pytest.raises(RuntimeError):
+        model.backward(torch.zeros_like(d))
+    # the last convolution with the head, through each conv form, against the
+    # separate pair: d1b's input is the stored d1a activation of the first pass
+    lib = gpu
+    a, p = model.a, model.p
+    model.forward(d, cond)
+    torch.cuda.synchronize()
+    for algo in ((1, 2, 3) if D >= 64 else (0, 3)):
+        prob = torch.full_like(model.prob, float("nan"))
+        c = lib.UnetConv(B=B, D=D, cin=16, cout=16, c0=16, o0=16, x0=_vp(a["d1a"]), w=_vp(model.wb["d1b"]),
+                         bias=_fp(p["d1b.b"]), relu=1, y0=None, algo=algo, head=_fp(p["head.w"]), prob=_fp(prob))
+        lib.call("cm21_unet_conv3_fwd", ctypes.byref(c), lib.stream_handle())
+        torch.cuda.synchronize()
+        print(f"algo {algo}: max |diff| {float((prob - p_sep).abs().max()):.3g}", flush=True)
+        assert torch.equal(prob, p_sep), f"algo {algo}"
