@@ -319,8 +319,73 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
     constexpr int PDC = 8 / NT < 2 ? 2 : (8 / NT > 6 ? 6 : 8 / NT);  // ring of at most 32 VGPRs
     constexpr int PDW = 24 / (MW * NT) < 2 ? 2 : (24 / (MW * NT) > PDC ? PDC : 24 / (MW * NT));
     constexpr int PD = KC < PDW ? KC : PDW;
+    // dy reuse (16-channel chunks, each wave owning a whole z-plane of
+    // lines): the K slices are regrouped into five families of tap pairs
+    // whose members differ only in dy,
+    //   f = 0, 1, 2: taps (dz=f, dy, dx=0) | (f, dy, 1)
+    //   f = 3:       taps (0, dy, 2)       | (1, dy, 2)
+    //   f = 4:       tap  (2, dy, 2)       | zero padding,
+    // (lane groups 0-1 take the first tap, 2-3 the second), so one LDS
+    // fragment of input line L feeds output lines L, L-1 and L-2 with the
+    // weights of dy = 0, 1, 2: 5 x (MW + 2) LDS reads per chunk instead of
+    // 14 x MW, for one padded slice more of MFMAs.
+    constexpr bool REUSE = CH == 16 && NT <= 2 && MW == TY;
+    const bool hi = grp >= 2;
+    const int lane_a = ((lz * HY) * HX + row) * CP + (grp & 1) * 8;
+    const int hi_ax = hi ? CP : 0, hi_az = hi ? HY * HX * CP : 0;
+    const bf16_t* wl0 = a.w + (size_t)row * K + (grp & 1) * 8;
+    const int hi_wx = hi ? CIN : 0, hi_wz = hi ? 9 * CIN : 0;
+    auto fam_w = [&](int c, int f, bf16x8 (&wq)[3][NT]) {
+        const int dz = f < 3 ? f : (f == 3 ? 0 : 2), dx = f < 3 ? 0 : 2;
+        const bool ok = !(f == 4 && hi);
+        const int hw = f < 3 ? hi_wx : (f == 3 ? hi_wz : 0);
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                const int tap = dz * 9 + dy * 3 + dx;
+                const u32x4 v = *(const u32x4*)(wl0 + n * 16 * K + tap * CIN + c * CH + hw);
+                wq[dy][n] = __builtin_bit_cast(bf16x8, ok ? v : zero);
+            }
+    };
+    auto fam_a = [&](int f, int L) -> bf16x8 {
+        const int dz = f < 3 ? f : (f == 3 ? 0 : 2), dx = f < 3 ? 0 : 2;
+        const bool ok = !(f == 4 && hi);
+        const int ha = f < 3 ? hi_ax : (f == 3 ? hi_az : 0);
+        const u32x4 v = *(const u32x4*)(tile + lane_a + ((dz * HY + L) * HX + dx) * CP + ha);
+        return __builtin_bit_cast(bf16x8, ok ? v : zero);
+    };
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
+        if constexpr (REUSE) {
+            bf16x8 wq[2][3][NT];
+            fam_w(c, 0, wq[0]);  // in flight during the tile load
+            if (c > 0) __syncthreads();
+            load_tile(c * CH);
+            __syncthreads();
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                if (f + 1 < 5) fam_w(c, f + 1, wq[(f + 1) & 1]);
+                bf16x8 acur = fam_a(f, 0), anext;
+#pragma unroll
+                for (int L = 0; L < MW + 2; ++L) {
+                    if (L + 1 < MW + 2) anext = fam_a(f, L + 1);
+#pragma unroll
+                    for (int dy = 0; dy < 3; ++dy) {
+                        const int m = L - dy;
+                        if (m >= 0 && m < MW) {
+#pragma unroll
+                            for (int n = 0; n < NT; ++n)
+                                acc[m][n] =
+                                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(wq[f & 1][dy][n], acur, acc[m][n], 0, 0, 0);
+                        }
+                    }
+                    acur = anext;
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            continue;
+        }
         bf16x8 bq[PD][NT], acur[MW], anext[MW];
 #pragma unroll
         for (int j = 0; j < PD; ++j) load_b(c, j, bq[j]);  // in flight during the tile load

@@ -363,11 +363,11 @@ def test_pipeline_validates_and_falls_back(gpu):
 
 @pytest.mark.parametrize("D", [64, 128])
 def test_fused_head_matches_unfused(gpu, D):
-    """Inference fuses the head into the last convolution; the probabilities
-    equal those of the separate head kernel bit for bit (both sum the logit
-    in the same order), every conv form that carries the head agrees, and
-    backward() refuses to run on the activations the fused pass did not
-    store."""
+    """Inference fuses the head into the last convolution: with the same conv
+    form the probabilities equal those of the separate head kernel bit for bit
+    (both sum the logit in the same order); the model's fused and stored
+    passes agree to bf16 rounding; backward() refuses to run on the
+    activations the fused pass did not store."""
     sur = pkg("surrogate")
     B = 1
     P = sur.init_params(3)
@@ -381,21 +381,32 @@ def test_fused_head_matches_unfused(gpu, D):
     p_sep = model.forward(d, cond).clone()
     p_fused = model.forward(d, cond, keep=False).clone()
     torch.cuda.synchronize()
-    print(f"D={D}: max |diff| {float((p_sep - p_fused).abs().max()):.3g}", flush=True)
-    assert torch.equal(p_sep, p_fused)
+    diff = (p_sep - p_fused).abs()
+    print(f"D={D}: max |diff| {float(diff.max()):.3g} mean {float(diff.mean()):.3g}", flush=True)
+    # the stored-activation pass and the fused pass may run the last conv in
+    # different forms (K summed in another order): a bf16 rounding of an
+    # activation may flip, nothing more
+    assert float(diff.max()) < 1e-2 and float(diff.mean()) < 1e-5
     with pytest.raises(RuntimeError):
         model.backward(torch.zeros_like(d))
-    # the last convolution with the head, through each conv form, against the
-    # separate pair: d1b's input is the stored d1a activation of the first pass
+    # every conv form that carries the head against the same form's stored
+    # output through the separate head kernel (the forms accumulate K in
+    # different orders, so each is compared with itself)
     lib = gpu
     a, p = model.a, model.p
     model.forward(d, cond)
-    torch.cuda.synchronize()
     for algo in ((1, 2, 3) if D >= 64 else (0, 3)):
+        y = torch.empty_like(a["d1"])
+        ref = torch.full_like(model.prob, float("nan"))
         prob = torch.full_like(model.prob, float("nan"))
-        c = lib.UnetConv(B=B, D=D, cin=16, cout=16, c0=16, o0=16, x0=_vp(a["d1a"]), w=_vp(model.wb["d1b"]),
-                         bias=_fp(p["d1b.b"]), relu=1, y0=None, algo=algo, head=_fp(p["head.w"]), prob=_fp(prob))
+        common = dict(B=B, D=D, cin=16, cout=16, c0=16, o0=16, x0=_vp(a["d1a"]), w=_vp(model.wb["d1b"]),
+                      bias=_fp(p["d1b.b"]), relu=1, algo=algo)
+        c = lib.UnetConv(y0=_vp(y), **common)
+        lib.call("cm21_unet_conv3_fwd", ctypes.byref(c), lib.stream_handle())
+        lib.call("cm21_unet_head", _vp(y), _fp(p["head.w"]), _fp(p["head.b"]), None, _fp(ref), None, None, None,
+                 B, D, None, 0, lib.stream_handle())
+        c = lib.UnetConv(y0=None, head=_fp(p["head.w"]), prob=_fp(prob), **common)
         lib.call("cm21_unet_conv3_fwd", ctypes.byref(c), lib.stream_handle())
         torch.cuda.synchronize()
-        print(f"algo {algo}: max |diff| {float((prob - p_sep).abs().max()):.3g}", flush=True)
-        assert torch.equal(prob, p_sep), f"algo {algo}"
+        print(f"algo {algo}: max |diff| {float((prob - ref).abs().max()):.3g}", flush=True)
+        assert torch.equal(prob, ref), f"algo {algo}"
