@@ -707,14 +707,151 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(WgradArgs a) {
     }
 }
 
-// Fixed-order sum over blocks of n-float partials: out[i] = sum_b partial[b][i].
-__global__ void k_sum_blocks(const float* __restrict__ partial, int nblk, int n, float* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float s = 0.0f;
-    for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * n + i];
-    out[i] = s;
+// Matrix-core weight gradient, dW[co][tap][ci] = sum_v g'[v][co] x[v + off(tap)][ci]
+// (g' = g where the forward output is positive), on v_mfma_f32_16x16x32_bf16
+// with K = 32 voxels of an x-line.  Grid: (voxel-slab blocks, 16-input-channel
+// chunk x dz).  Voxel tiles of 2 x 2 lines of 32 voxels; the masked gradient
+// (all cout channels) and the chunk's input (the 2 planes at dz, 4 rows, 34
+// voxels) are written to LDS channel-major, the input in three copies shifted
+// by dx = 0, 1, 2, so that every fragment (8 consecutive voxels of one
+// channel) is one aligned 16-byte read.  Wave w owns line w of every tile and
+// keeps the 9 (dy, dx) taps x cout/16 accumulator tiles over all of the
+// block's tiles; the four waves are added in a fixed order at the end, and
+// the blocks by k_sum_blocks.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_conv3_wgrad_mfma(WgradArgs a) {
+    constexpr int NTC = COUT / 16, NTI = CIN / 16, XT = 32, XH = XT + 2, HY = 4;
+    constexpr int GS = 4 * XT + 8;           // gs channel stride (bf16), padded against bank conflicts
+    constexpr int XS = 2 * HY * XT + 8;      // xs channel stride
+    static_assert(CIN % 16 == 0 && COUT % 16 == 0, "channel counts");
+    __shared__ __attribute__((aligned(16))) bf16_t xs[3 * 16 * XS];
+    __shared__ __attribute__((aligned(16))) bf16_t gs[COUT * GS];
+    const int D = a.D;
+    const int cit = blockIdx.y % NTI, dz = blockIdx.y / NTI;
+    const int t = threadIdx.x, lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int row = lane & 15, grp = lane >> 4;
+    const int lz = wave >> 1, ly = wave & 1;
+    const int nx = D / XT, ny = D / 2, nz = D / 2;
+    const int ntiles = a.B * nz * ny * nx;
+    // the chunk's channels come from x0 or x1 (c0 is a multiple of 16)
+    const int cb = cit * 16;
+    const bool from0 = cb < a.c0;
+    const bf16_t* xsrc = from0 ? a.x0 : a.x1;
+    const int xstride = from0 ? a.c0 : CIN - a.c0;
+    const int xoff = from0 ? cb : cb - a.c0;
+    floatx4 acc[9][NTC];
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+#pragma unroll
+        for (int n = 0; n < NTC; ++n) acc[q][n] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int s = 0; s < a.tiles_per_block; ++s) {
+        const int tile = blockIdx.x * a.tiles_per_block + s;
+        if (tile >= ntiles) break;  // uniform
+        const int x0 = (tile % nx) * XT;
+        const int y0 = ((tile / nx) % ny) * 2;
+        const int z0 = ((tile / (nx * ny)) % nz) * 2;
+        const int b = tile / (nx * ny * nz);
+        const size_t vb = (size_t)b * D * D * D;
+        __syncthreads();  // the previous tile is consumed
+        // masked output gradient: gs[co][line][x], line = 2 lz + ly
+        for (int i = t; i < 4 * XT * (COUT / 8); i += 256) {
+            const int ch = (i % (COUT / 8)) * 8, v = i / (COUT / 8);
+            const int x = v % XT, l = v / XT;
+            const size_t g = vb + ((size_t)(z0 + (l >> 1)) * D + (y0 + (l & 1))) * D + x0 + x;
+            u32x4 q = *(const u32x4*)(a.g + g * COUT + ch);
+            if (a.mask) q = mask8(q, *(const u32x4*)(a.mask + g * COUT + ch));
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                gs[(ch + 2 * k) * GS + l * XT + x] = (bf16_t)(q[k] & 0xffffu);
+                gs[(ch + 2 * k + 1) * GS + l * XT + x] = (bf16_t)(q[k] >> 16);
+            }
+        }
+        // the chunk's input: xs[dx][ci][pz][hy][xd] = x(z0 + pz + dz - 1, y0 + hy - 1, x0 + xd + dx - 1)
+        for (int i = t; i < 2 * HY * XH * 2; i += 256) {
+            const int half = i & 1, v = i >> 1;
+            const int xi = v % XH, r = v / XH;
+            const int pz = r / HY, hy = r % HY;
+            const size_t g =
+                vb + ((size_t)wrap(z0 + pz + dz - 1, D) * D + wrap(y0 + hy - 1, D)) * D + wrap(x0 + xi - 1, D);
+            const u32x4 q = *(const u32x4*)(xsrc + g * xstride + xoff + half * 8);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int xd = xi - c;
+                if (xd >= 0 && xd < XT) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int ci = half * 8 + 2 * k;
+                        xs[(c * 16 + ci) * XS + r * XT + xd] = (bf16_t)(q[k] & 0xffffu);
+                        xs[(c * 16 + ci + 1) * XS + r * XT + xd] = (bf16_t)(q[k] >> 16);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        bf16x8 ga[NTC];
+#pragma unroll
+        for (int n = 0; n < NTC; ++n)
+            ga[n] = __builtin_bit_cast(bf16x8, *(const u32x4*)(gs + (n * 16 + row) * GS + wave * XT + grp * 8));
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const bf16x8 xb = __builtin_bit_cast(
+                    bf16x8, *(const u32x4*)(xs + (dx * 16 + row) * XS + (lz * HY + ly + dy) * XT + grp * 8));
+#pragma unroll
+                for (int n = 0; n < NTC; ++n)
+                    acc[dy * 3 + dx][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ga[n], xb, acc[dy * 3 + dx][n], 0, 0, 0);
+            }
+    }
+    // fixed-order sum of the four waves, one tap at a time, into this block's
+    // partial (lane: co = 16 n + 4 grp + i, ci = cb + row)
+    float* red = (float*)xs;
+    float* out = a.partial + (size_t)blockIdx.x * COUT * 27 * CIN;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        __syncthreads();
+        if (wave > 0) {
+#pragma unroll
+            for (int n = 0; n < NTC; ++n) *(floatx4*)(red + (((wave - 1) * NTC + n) * 64 + lane) * 4) = acc[q][n];
+        }
+        __syncthreads();
+        if (wave == 0) {
+            const int tap = dz * 9 + q;
+#pragma unroll
+            for (int n = 0; n < NTC; ++n) {
+                floatx4 sum = acc[q][n];
+#pragma unroll
+                for (int w = 0; w < 3; ++w) sum += *(const floatx4*)(red + ((w * NTC + n) * 64 + lane) * 4);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) out[((size_t)(n * 16 + grp * 4 + i) * 27 + tap) * CIN + cb + row] = sum[i];
+            }
+        }
+    }
 }
+
+// Fixed-order sum over blocks of n-float partials: out[i] = sum_b partial[b][i].
+// A block of 256 threads covers 32 outputs; thread group g (of 8) adds the
+// partials b = g, g + 8, ... in turn, then the 8 group sums are added in
+// order (deterministic, and 8 short dependent chains instead of one long one).
+constexpr int kSumOut = 32;
+__global__ __launch_bounds__(256) void k_sum_blocks(const float* __restrict__ partial, int nblk, int n,
+                                                    float* __restrict__ out) {
+    __shared__ float red[8][kSumOut];
+    const int j = threadIdx.x % kSumOut, g = threadIdx.x / kSumOut;
+    const int i = blockIdx.x * kSumOut + j;
+    float s = 0.0f;
+    if (i < n)
+        for (int b = g; b < nblk; b += 8) s += partial[(size_t)b * n + i];
+    red[g][j] = s;
+    __syncthreads();
+    if (g == 0 && i < n) {
+        float r = red[0][j];
+        for (int q = 1; q < 8; ++q) r += red[q][j];
+        out[i] = r;
+    }
+}
+inline unsigned sum_blocks_grid(int n) { return (unsigned)((n + kSumOut - 1) / kSumOut); }
 
 // Per-sample channel sums of a (masked) gradient over a chunk of one
 // sample's voxels per block: partial[b][blk][c].  Each thread reads 16-byte
@@ -755,18 +892,26 @@ __global__ __launch_bounds__(256) void k_channel_partials(const bf16_t* __restri
 
 // per_sample[b][c] = sum_blk partial[b][blk][c]; total[c] = sum_b per_sample[b][c]
 // (either output may be NULL).
-__global__ void k_channel_final(const float* __restrict__ partial, int B, int nblk, int C,
-                                float* __restrict__ per_sample, float* __restrict__ total) {
-    const int c = threadIdx.x;
-    if (c >= C) return;
+// One block per channel, one thread per partial block (nblk <= 128): for each
+// sample the 128 partials are added by a fixed butterfly within each wave and
+// the two wave sums in order; the samples in order.
+__global__ __launch_bounds__(128) void k_channel_final(const float* __restrict__ partial, int B, int nblk, int C,
+                                                       float* __restrict__ per_sample, float* __restrict__ total) {
+    __shared__ float ws[2];
+    const int c = blockIdx.x, k = threadIdx.x, lane = k & 63, w = k >> 6;
     float tot = 0.0f;
     for (int b = 0; b < B; ++b) {
-        float s = 0.0f;
-        for (int k = 0; k < nblk; ++k) s += partial[((size_t)b * nblk + k) * C + c];
-        if (per_sample) per_sample[b * C + c] = s;
+        float v = k < nblk ? partial[((size_t)b * nblk + k) * C + c] : 0.0f;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) ws[w] = v;
+        __syncthreads();
+        const float s = ws[0] + ws[1];
+        __syncthreads();
+        if (k == 0 && per_sample) per_sample[b * C + c] = s;
         tot += s;
     }
-    if (total) total[c] = tot;
+    if (k == 0 && total) total[c] = tot;
 }
 
 // ------------------------------------------------------------------ pooling
@@ -1280,15 +1425,21 @@ template <int CIN, int COUT>
 void launch_wgrad(const WgradArgs& a, int nblk, hipStream_t st) {
     k_conv3_wgrad<CIN, COUT><<<dim3(nblk, 9), 256, 0, st>>>(a);
 }
+template <int CIN, int COUT>
+void launch_wgrad_mfma(const WgradArgs& a, int nblk, hipStream_t st) {
+    k_conv3_wgrad_mfma<CIN, COUT><<<dim3(nblk, 3 * (CIN / 16)), 256, 0, st>>>(a);
+}
 struct WgradShape {
     int cin, cout;
-    WgradFn fn;
+    WgradFn fn;    // fp32 VALU, voxel tiles 2 x 2 x 16 (any D)
+    WgradFn mfma;  // matrix cores, voxel tiles 2 x 2 x 32 (D a multiple of 32, c0 a multiple of 16)
 };
+#define CM21_WGRAD(CI, CO) {CI, CO, launch_wgrad<CI, CO>, launch_wgrad_mfma<CI, CO>}
 const WgradShape kWgradShapes[] = {
-    {16, 16, launch_wgrad<16, 16>}, {16, 32, launch_wgrad<16, 32>}, {32, 16, launch_wgrad<32, 16>},
-    {32, 32, launch_wgrad<32, 32>}, {32, 64, launch_wgrad<32, 64>}, {64, 32, launch_wgrad<64, 32>},
-    {64, 64, launch_wgrad<64, 64>},
+    CM21_WGRAD(16, 16), CM21_WGRAD(16, 32), CM21_WGRAD(32, 16), CM21_WGRAD(32, 32),
+    CM21_WGRAD(32, 64), CM21_WGRAD(64, 32), CM21_WGRAD(64, 64),
 };
+#undef CM21_WGRAD
 const WgradShape* wgrad_shape(int cin, int cout) {
     for (const auto& s : kWgradShapes)
         if (s.cin == cin && s.cout == cout) return &s;
@@ -1303,7 +1454,8 @@ int channel_sums(const void* g, const void* mask, int C, size_t vps, int B, floa
                  float* per_sample, hipStream_t st) {
     k_channel_partials<<<dim3(kChanBlocks, B), 256, 0, st>>>((const bf16_t*)g, (const bf16_t*)mask, C, vps, cpart);
     CM21_LAUNCH_CHECK("k_channel_partials");
-    k_channel_final<<<1, 64, 0, st>>>(cpart, B, kChanBlocks, C, per_sample, total);
+    static_assert(kChanBlocks <= 128, "one thread per partial block");
+    k_channel_final<<<C, 128, 0, st>>>(cpart, B, kChanBlocks, C, per_sample, total);
     CM21_LAUNCH_CHECK("k_channel_final");
     return CM21_OK;
 }
@@ -1388,7 +1540,10 @@ extern "C" int cm21_unet_conv3_wgrad(const cm21_unet_wgrad* w, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     float* partial = (float*)w->work;
     float* cpart = (float*)((char*)w->work + align_up((size_t)kWgradBlocks * w->cout * 27 * w->cin * sizeof(float), 256));
-    const int ntiles = w->B * (w->D / 2) * (w->D / 2) * (w->D / 16);
+    // the matrix-core form needs x-lines of 32 voxels and whole 16-channel
+    // chunks per input tensor; the VALU form takes the rest
+    const bool mfma = w->D % 32 == 0 && w->c0 % 16 == 0;
+    const int ntiles = w->B * (w->D / 2) * (w->D / 2) * (w->D / (mfma ? 32 : 16));
     const int tpb = (ntiles + kWgradBlocks - 1) / kWgradBlocks;
     const int nblk = (ntiles + tpb - 1) / tpb;
     WgradArgs a{};
@@ -1401,10 +1556,10 @@ extern "C" int cm21_unet_conv3_wgrad(const cm21_unet_wgrad* w, void* stream) {
     a.mask = (const bf16_t*)w->mask;
     a.partial = partial;
     a.tiles_per_block = tpb;
-    s->fn(a, nblk, st);
+    (mfma ? s->mfma : s->fn)(a, nblk, st);
     CM21_LAUNCH_CHECK("k_conv3_wgrad");
     const int n = w->cout * 27 * w->cin;
-    k_sum_blocks<<<(n + 255) / 256, 256, 0, st>>>(partial, nblk, n, w->dw);
+    k_sum_blocks<<<sum_blocks_grid(n), 256, 0, st>>>(partial, nblk, n, w->dw);
     CM21_LAUNCH_CHECK("k_sum_blocks");
     if (w->db) CM21_TRY(channel_sums(w->g, w->mask, w->cout, (size_t)w->D * w->D * w->D, w->B, cpart, w->db, nullptr, st));
     return CM21_OK;
@@ -1452,7 +1607,7 @@ int up_bwd_t(const void* x, const void* dy, const void* wk, void* dx, float* dw,
     k_up_wgrad<CIN, COUT><<<nblk, 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)dy, part, B, D, vpb);
     CM21_LAUNCH_CHECK("k_up_wgrad");
     const int n = COUT * 8 * CIN;
-    k_sum_blocks<<<(n + 255) / 256, 256, 0, st>>>(part, nblk, n, dw);
+    k_sum_blocks<<<sum_blocks_grid(n), 256, 0, st>>>(part, nblk, n, dw);
     CM21_LAUNCH_CHECK("k_sum_blocks");
     const size_t vps = (size_t)8 * D * D * D;
     CM21_TRY(channel_sums(dy, nullptr, COUT, vps, B, cpart, db, nullptr, st));
@@ -1539,7 +1694,7 @@ extern "C" int cm21_unet_in_wgrad(const float* delta, const float* cond, const v
     const int nblk = (ntiles + tpb - 1) / tpb;
     k_in_wgrad<16><<<nblk, 256, 0, st>>>(delta, (const bf16_t*)g, (const bf16_t*)y, part, B, D, delta_scale, tpb);
     CM21_LAUNCH_CHECK("k_in_wgrad");
-    k_sum_blocks<<<(16 * 27 + 255) / 256, 256, 0, st>>>(part, nblk, 16 * 27, dwd);
+    k_sum_blocks<<<sum_blocks_grid(16 * 27), 256, 0, st>>>(part, nblk, 16 * 27, dwd);
     CM21_LAUNCH_CHECK("k_sum_blocks");
     CM21_TRY(channel_sums(g, y, 16, (size_t)D * D * D, B, cpart, nullptr, gsum, st));
     k_in_finish<<<(16 * 27 + 255) / 256, 256, 0, st>>>(dwd, gsum, cond, dw, db, B, 16);
@@ -1566,7 +1721,7 @@ extern "C" int cm21_unet_head(const void* x, const float* w, const float* bias, 
                                             (float)(1.0 / (double)nv));
     CM21_LAUNCH_CHECK("k_head");
     if (target) {
-        k_sum_blocks<<<1, 256, 0, st>>>(part, kHeadBlocks, 17, dwb);
+        k_sum_blocks<<<sum_blocks_grid(17), 256, 0, st>>>(part, kHeadBlocks, 17, dwb);
         CM21_LAUNCH_CHECK("k_sum_blocks");
         k_sum_f64<<<1, 256, 0, st>>>(lpart, kHeadBlocks, 1.0 / (double)nv, loss);
         CM21_LAUNCH_CHECK("k_sum_f64");

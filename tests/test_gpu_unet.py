@@ -102,9 +102,13 @@ def test_conv3_fwd(gpu, cin, cout, D, split):
             _close_bf16(y1, ref[..., o0:], f"y1 algo {algo}")
 
 
-@pytest.mark.parametrize("cin,cout,D", [(16, 16, 32), (32, 16, 16), (64, 32, 16), (32, 64, 16), (16, 16, 64),
-                                        (16, 32, 64), (32, 16, 64)])
-def test_conv3_input_and_weight_gradients(gpu, cin, cout, D):
+# weight gradient: fp32 VALU form at D = 16, matrix-core form at D >= 32
+# (split: the input arrives as two tensors of cin/2 channels)
+@pytest.mark.parametrize("cin,cout,D,split", [(16, 16, 32, False), (32, 16, 16, False), (64, 32, 16, False),
+                                              (32, 64, 16, False), (16, 16, 64, False), (16, 32, 64, False),
+                                              (32, 16, 64, True), (64, 32, 32, True), (32, 64, 32, False),
+                                              (64, 64, 32, False)])
+def test_conv3_input_and_weight_gradients(gpu, cin, cout, D, split):
     """Input gradient = the MFMA conv on the masked output gradient with the
     packed transposed weights; weight/bias gradients from the wgrad kernel;
     both against autograd of the oracle conv."""
@@ -123,8 +127,11 @@ def test_conv3_input_and_weight_gradients(gpu, cin, cout, D):
     db = torch.empty(cout, dtype=torch.float32, device="cuda")
     work = torch.empty(lib.query("cm21_unet_wgrad_workspace_bytes", B, D, cin, cout), dtype=torch.uint8,
                        device="cuda")
-    wg = lib.UnetWgrad(B=B, D=D, cin=cin, cout=cout, c0=cin, x0=_vp(x), g=_vp(g), mask=_vp(m), dw=_fp(dw),
-                       db=_fp(db), work=_vp(work), work_bytes=work.numel())
+    c0 = cin // 2 if split else cin
+    xa, xb = (x[..., :c0].contiguous(), x[..., c0:].contiguous()) if split else (x, None)
+    print(f"grad cin={cin} cout={cout} D={D} split={split}", flush=True)
+    wg = lib.UnetWgrad(B=B, D=D, cin=cin, cout=cout, c0=c0, x0=_vp(xa), x1=_vp(xb), g=_vp(g), mask=_vp(m),
+                       dw=_fp(dw), db=_fp(db), work=_vp(work), work_bytes=work.numel())
     lib.call("cm21_unet_conv3_wgrad", ctypes.byref(wg), lib.stream_handle())
     torch.cuda.synchronize()
     assert torch.equal(wb.cpu(), w32.cpu().to(torch.bfloat16))
