@@ -1058,6 +1058,83 @@ __global__ void k_pack_up(const float* __restrict__ w, bf16_t* __restrict__ wn, 
 
 // dW[co][s][ci] = sum_v dy[2v+s][co] x[v][ci]: block partials over voxel
 // ranges; each thread owns 4 x 4 (co, ci) for all 8 s; sets split voxels.
+// Matrix-core weight gradient of the transposed conv on
+// v_mfma_f32_16x16x4f32: dW[co][s][ci] = sum_v dy[2v+s][co] x[v][ci].  K = 4
+// voxels per product, one per lane group, so both operands come straight
+// from the channels-last tensors (bf16 -> fp32 is exact): lane (row, grp)
+// supplies dy[out(voxel grp, s)][16 cot + row] and x[voxel grp][16 cit + row].
+// Grid: (blocks over x-lines, groups of SPW sub-positions); wave w takes the
+// 4-voxel steps 4 w, 4 w + 16, ... of each line and keeps SPW x cout/16 x
+// cin/16 accumulator tiles; the waves are added in a fixed order at the end.
+template <int CIN, int COUT, int SPW>
+__global__ __launch_bounds__(256) void k_up_wgrad_mfma(const bf16_t* __restrict__ x, const bf16_t* __restrict__ dy,
+                                                       float* __restrict__ partial, int B, int D,
+                                                       int lines_per_block) {
+    constexpr int NTC = COUT / 16, NTI = CIN / 16;
+    static_assert(CIN % 16 == 0 && COUT % 16 == 0 && 8 % SPW == 0, "shapes");
+    __shared__ floatx4 red[3][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int row = lane & 15, grp = lane >> 4;
+    const int s0 = blockIdx.y * SPW;
+    const size_t D2 = 2 * (size_t)D;
+    floatx4 acc[SPW][NTC][NTI];
+#pragma unroll
+    for (int a = 0; a < SPW; ++a)
+#pragma unroll
+        for (int n = 0; n < NTC; ++n)
+#pragma unroll
+            for (int m = 0; m < NTI; ++m) acc[a][n][m] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int nlines = B * D * D;
+    const int L0 = blockIdx.x * lines_per_block;
+    const int L1 = L0 + lines_per_block < nlines ? L0 + lines_per_block : nlines;
+    for (int L = L0; L < L1; ++L) {
+        const size_t yy = L % D, zz = (L / D) % D, b = L / (D * D);
+        size_t ob[SPW];
+#pragma unroll
+        for (int a = 0; a < SPW; ++a) {
+            const int s = s0 + a;
+            ob[a] = ((b * D2 + 2 * zz + (s >> 2)) * D2 + 2 * yy + ((s >> 1) & 1)) * D2 + (s & 1);
+        }
+        for (int xb = 4 * wave; xb < D; xb += 16) {
+            const int xx = xb + grp;
+            float bx[NTI];
+#pragma unroll
+            for (int m = 0; m < NTI; ++m) bx[m] = bf2f(x[((size_t)L * D + xx) * CIN + m * 16 + row]);
+#pragma unroll
+            for (int a = 0; a < SPW; ++a) {
+                const size_t u = ob[a] + 2 * (size_t)xx;
+#pragma unroll
+                for (int n = 0; n < NTC; ++n) {
+                    const float av = bf2f(dy[u * COUT + n * 16 + row]);
+#pragma unroll
+                    for (int m = 0; m < NTI; ++m)
+                        acc[a][n][m] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bx[m], acc[a][n][m], 0, 0, 0);
+                }
+            }
+        }
+    }
+    float* out = partial + (size_t)blockIdx.x * COUT * 8 * CIN;
+#pragma unroll
+    for (int a = 0; a < SPW; ++a)
+#pragma unroll
+        for (int n = 0; n < NTC; ++n)
+#pragma unroll
+            for (int m = 0; m < NTI; ++m) {
+                __syncthreads();
+                if (wave > 0) red[wave - 1][lane] = acc[a][n][m];
+                __syncthreads();
+                if (wave == 0) {
+                    floatx4 sum = acc[a][n][m];
+#pragma unroll
+                    for (int w = 0; w < 3; ++w) sum += red[w][lane];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        out[((size_t)(n * 16 + grp * 4 + i) * 8 + s0 + a) * CIN + m * 16 + row] = sum[i];
+                }
+            }
+}
+
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void k_up_wgrad(const bf16_t* __restrict__ x, const bf16_t* __restrict__ dy,
                                                   float* __restrict__ partial, int B, int D, int vox_per_block) {
@@ -1188,62 +1265,72 @@ __global__ __launch_bounds__(256) void k_in_fwd(const float* __restrict__ delta,
     }
 }
 
-// partial[blk][co][t] = sum_v g'[v][co] * scale * delta[v + off_t] with
-// g' = g masked by y > 0, over voxel tiles of 4 x 4 x 16 (periodic halo of
-// delta and the masked gradient in LDS); each thread owns (co, tap) pairs.
-template <int COUT>
-__global__ __launch_bounds__(256) void k_in_wgrad(const float* __restrict__ delta, const bf16_t* __restrict__ g,
-                                                  const bf16_t* __restrict__ y, float* __restrict__ partial, int B,
-                                                  int D, float scale, int tiles_per_block) {
-    constexpr int TZ = 4, TY = 4, HZ = 6, HY = 6, HX = 18, NV = TZ * TY * 16, NP = COUT * 27;
-    constexpr int PP = (NP + 255) / 256;
-    __shared__ float dt[HZ * HY * HX];
-    __shared__ float gt[NV * COUT];
-    const int t = threadIdx.x;
-    const int nx = D / 16, ny = D / TY, nz = D / TZ;
-    const int ntiles = B * nz * ny * nx;
-    float acc[PP];
+// Input-layer weight gradient of the density channel on the fp32 matrix
+// cores (v_mfma_f32_16x16x4f32): dwd[co][t] = sum_v g'[v][co] scale
+// delta[v + off(t)] (g' = g where the layer's output is positive).  K = 4
+// voxels per product, one per lane group, so both operands come straight from
+// memory: lane (row, grp) supplies g'[voxel grp][channel row] and the scaled
+// density at voxel grp shifted by tap 16 j + row (two N tiles cover the 27
+// taps; tap rows fetched per x-line).  Grid: blocks over x-lines; wave w takes
+// the 4-voxel steps 4 w, 4 w + 16, ... of each line; the waves are added in a
+// fixed order at the end.
+__global__ __launch_bounds__(256) void k_in_wgrad_mfma(const float* __restrict__ delta, const bf16_t* __restrict__ g,
+                                                       const bf16_t* __restrict__ y, float* __restrict__ partial, int B,
+                                                       int D, float scale, int lines_per_block) {
+    __shared__ floatx4 red[3][2][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int row = lane & 15, grp = lane >> 4;
+    int tdz[2], tdy[2], tdx[2];
+    bool tok[2];
 #pragma unroll
-    for (int q = 0; q < PP; ++q) acc[q] = 0.0f;
-    for (int s = 0; s < tiles_per_block; ++s) {
-        const int tile = blockIdx.x * tiles_per_block + s;
-        if (tile >= ntiles) break;  // uniform
-        const int xt0 = (tile % nx) * 16;
-        const int yt0 = ((tile / nx) % ny) * TY;
-        const int zt0 = ((tile / (nx * ny)) % nz) * TZ;
-        const int b = tile / (nx * ny * nz);
-        __syncthreads();
-        for (int i = t; i < HZ * HY * HX; i += 256) {
-            const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
-            dt[i] = scale * delta[(((size_t)b * D + wrap(zt0 + hz - 1, D)) * D + wrap(yt0 + hy - 1, D)) * D +
-                                  wrap(xt0 + hx - 1, D)];
-        }
-        for (int i = t; i < NV * COUT; i += 256) {
-            const int co = i % COUT, v = i / COUT;
-            const int lx = v % 16, ly = (v / 16) % TY, lz = v / (16 * TY);
-            const size_t idx = ((((size_t)b * D + zt0 + lz) * D + yt0 + ly) * D + xt0 + lx) * COUT + co;
-            gt[i] = pos_bf(y[idx]) ? bf2f(g[idx]) : 0.0f;
-        }
-        __syncthreads();
+    for (int j = 0; j < 2; ++j) {
+        const int t = 16 * j + row;
+        tok[j] = t < 27;
+        const int tt = tok[j] ? t : 0;
+        tdz[j] = tt / 9 - 1;
+        tdy[j] = (tt / 3) % 3 - 1;
+        tdx[j] = tt % 3 - 1;
+    }
+    floatx4 acc[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
+    const int nlines = B * D * D;
+    const int L0 = blockIdx.x * lines_per_block;
+    const int L1 = L0 + lines_per_block < nlines ? L0 + lines_per_block : nlines;
+    for (int L = L0; L < L1; ++L) {
+        const int yy = L % D, zz = (L / D) % D, b = L / (D * D);
+        const float* drow[2];
 #pragma unroll
-        for (int q = 0; q < PP; ++q) {
-            const int p = t + q * 256;
-            if (p < NP) {
-                const int co = p / 27, tap = p % 27;
-                const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-                float a = acc[q];
-                for (int v = 0; v < NV; ++v) {
-                    const int lx = v % 16, ly = (v / 16) % TY, lz = v / (16 * TY);
-                    a = fmaf(gt[v * COUT + co], dt[((lz + dz) * HY + (ly + dy)) * HX + lx + dx], a);
-                }
-                acc[q] = a;
+        for (int j = 0; j < 2; ++j)
+            drow[j] = delta + (((size_t)b * D + wrap(zz + tdz[j], D)) * D + wrap(yy + tdy[j], D)) * D;
+        const size_t lbase = (size_t)L * D;
+        for (int xb = 4 * wave; xb < D; xb += 16) {
+            const int xx = xb + grp;
+            const size_t idx = (lbase + xx) * 16 + row;
+            const float av = pos_bf(y[idx]) ? bf2f(g[idx]) : 0.0f;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const float bv = tok[j] ? scale * drow[j][wrap(xx + tdx[j], D)] : 0.0f;
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[j], 0, 0, 0);
             }
         }
     }
+    if (wave > 0) {
 #pragma unroll
-    for (int q = 0; q < PP; ++q) {
-        const int p = t + q * 256;
-        if (p < NP) partial[(size_t)blockIdx.x * NP + p] = acc[q];
+        for (int j = 0; j < 2; ++j) red[wave - 1][j][lane] = acc[j];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            floatx4 sum = acc[j];
+#pragma unroll
+            for (int w = 0; w < 3; ++w) sum += red[w][j][lane];
+            const int tap = 16 * j + row;
+            if (tap < 27) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) partial[(size_t)blockIdx.x * 16 * 27 + (grp * 4 + i) * 27 + tap] = sum[i];
+            }
+        }
     }
 }
 
@@ -1460,7 +1547,8 @@ int channel_sums(const void* g, const void* mask, int C, size_t vps, int B, floa
     return CM21_OK;
 }
 constexpr int kHeadBlocks = 1024;
-constexpr int kUpBlocksMax = 256;
+constexpr int kUpBlocksMax = 1024;  // line blocks of the transposed-conv weight gradient
+constexpr int kInBlocks = 1024;     // line blocks of the input-layer weight gradient
 
 bool valid_unet_dim(int D) { return D >= 16 && D <= 1024 && D % 16 == 0; }
 
@@ -1602,9 +1690,20 @@ int up_bwd_t(const void* x, const void* dy, const void* wk, void* dx, float* dw,
                                                                                (bf16_t*)dx, B, D);
         CM21_LAUNCH_CHECK("k_up_bwd_mfma");
     }
-    const int vpb = (int)((nv + kUpBlocksMax - 1) / kUpBlocksMax);
-    const int nblk = (int)((nv + vpb - 1) / vpb);
-    k_up_wgrad<CIN, COUT><<<nblk, 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)dy, part, B, D, vpb);
+    int nblk;
+    if (D % 4 == 0) {
+        // matrix cores; SPW sub-positions per block keep 64 accumulator registers
+        constexpr int SPW = (COUT / 16) * (CIN / 16) <= 2 ? 8 : 2;
+        const int nlines = B * D * D;
+        const int lpb = (nlines + kUpBlocksMax - 1) / kUpBlocksMax;
+        nblk = (nlines + lpb - 1) / lpb;
+        k_up_wgrad_mfma<CIN, COUT, SPW><<<dim3(nblk, 8 / SPW), 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)dy,
+                                                                             part, B, D, lpb);
+    } else {
+        const int vpb = (int)((nv + kUpBlocksMax - 1) / kUpBlocksMax);
+        nblk = (int)((nv + vpb - 1) / vpb);
+        k_up_wgrad<CIN, COUT><<<nblk, 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)dy, part, B, D, vpb);
+    }
     CM21_LAUNCH_CHECK("k_up_wgrad");
     const int n = COUT * 8 * CIN;
     k_sum_blocks<<<sum_blocks_grid(n), 256, 0, st>>>(part, nblk, n, dw);
@@ -1668,7 +1767,7 @@ extern "C" int cm21_unet_in_fwd(const float* delta, const float* cond, const flo
 
 extern "C" size_t cm21_unet_in_workspace_bytes(int B, int D) {
     if (B < 1 || !valid_unet_dim(D)) return 0;
-    return align_up((size_t)kWgradBlocks * 16 * 27 * sizeof(float), 256) +
+    return align_up((size_t)kInBlocks * 16 * 27 * sizeof(float), 256) +
            align_up((size_t)B * kChanBlocks * 16 * sizeof(float), 256) + align_up((size_t)B * 16 * sizeof(float), 256) +
            align_up((size_t)16 * 27 * sizeof(float), 256);
 }
@@ -1683,17 +1782,17 @@ extern "C" int cm21_unet_in_wgrad(const float* delta, const float* cond, const v
     hipStream_t st = (hipStream_t)stream;
     char* p = (char*)work;
     float* part = (float*)p;
-    p += align_up((size_t)kWgradBlocks * 16 * 27 * sizeof(float), 256);
+    p += align_up((size_t)kInBlocks * 16 * 27 * sizeof(float), 256);
     float* cpart = (float*)p;
     p += align_up((size_t)B * kChanBlocks * 16 * sizeof(float), 256);
     float* gsum = (float*)p;
     p += align_up((size_t)B * 16 * sizeof(float), 256);
     float* dwd = (float*)p;
-    const int ntiles = B * (D / 4) * (D / 4) * (D / 16);
-    const int tpb = (ntiles + kWgradBlocks - 1) / kWgradBlocks;
-    const int nblk = (ntiles + tpb - 1) / tpb;
-    k_in_wgrad<16><<<nblk, 256, 0, st>>>(delta, (const bf16_t*)g, (const bf16_t*)y, part, B, D, delta_scale, tpb);
-    CM21_LAUNCH_CHECK("k_in_wgrad");
+    const int nlines = B * D * D;
+    const int lpb = (nlines + kInBlocks - 1) / kInBlocks;
+    const int nblk = (nlines + lpb - 1) / lpb;
+    k_in_wgrad_mfma<<<nblk, 256, 0, st>>>(delta, (const bf16_t*)g, (const bf16_t*)y, part, B, D, delta_scale, lpb);
+    CM21_LAUNCH_CHECK("k_in_wgrad_mfma");
     k_sum_blocks<<<sum_blocks_grid(16 * 27), 256, 0, st>>>(part, nblk, 16 * 27, dwd);
     CM21_LAUNCH_CHECK("k_sum_blocks");
     CM21_TRY(channel_sums(g, y, 16, (size_t)D * D * D, B, cpart, nullptr, gsum, st));
