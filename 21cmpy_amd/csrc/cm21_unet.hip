@@ -966,6 +966,8 @@ __global__ void k_pool_bwd(const bf16_t* __restrict__ dy, bf16_t* __restrict__ d
 // Input gradient: K = (s, co), N = ci: dx[v][ci] = sum_k wk[ci][k] dy[2v+s][co],
 // with wk bf16 [CIN][8*COUT].  A and B fragments are 16-byte loads of 8
 // consecutive channels; each wave walks its 16-voxel row segments.
+constexpr int kUpSegs = 4;  // 16-voxel input segments per wave of the transposed conv
+
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void k_up_fwd_mfma(const bf16_t* __restrict__ x, const bf16_t* __restrict__ wn,
                                                      const float* __restrict__ bias, bf16_t* __restrict__ y, int B,
@@ -978,33 +980,38 @@ __global__ __launch_bounds__(256) void k_up_fwd_mfma(const bf16_t* __restrict__ 
     static_assert(CIN % 32 == 0 && COUT % 16 == 0, "channel counts");
     const int lane = threadIdx.x & 63, row = lane & 15, grp = lane >> 4;
     const size_t nseg = (size_t)B * D * D * D / 16;
-    const size_t seg = (size_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (seg >= nseg) return;  // whole wave
-    const size_t v0 = seg * 16;
-    bf16x8 av[KC];
+    const size_t seg0 = ((size_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * kUpSegs;
+    for (int sg = 0; sg < kUpSegs; ++sg) {
+        const size_t seg = seg0 + sg;
+        if (seg >= nseg) return;  // whole wave
+        const size_t v0 = seg * 16;
+        bf16x8 av[KC];
 #pragma unroll
-    for (int k = 0; k < KC; ++k) av[k] = __builtin_bit_cast(bf16x8, *(const u32x4*)(x + (v0 + row) * CIN + k * 32 + grp * 8));
-    const size_t D2 = 2 * (size_t)D;
-    // output voxel of this lane's input voxel at sub-position 0
-    const size_t v = v0 + row;
-    const size_t xx = v % D, yy = (v / D) % D, zz = (v / ((size_t)D * D)) % D, b = v / ((size_t)D * D * D);
-    const size_t u00 = ((b * D2 + 2 * zz) * D2 + 2 * yy) * D2 + 2 * xx;
+        for (int k = 0; k < KC; ++k)
+            av[k] = __builtin_bit_cast(bf16x8, *(const u32x4*)(x + (v0 + row) * CIN + k * 32 + grp * 8));
+        const size_t D2 = 2 * (size_t)D;
+        // output voxel of this lane's input voxel at sub-position 0
+        const size_t v = v0 + row;
+        const size_t xx = v % D, yy = (v / D) % D, zz = (v / ((size_t)D * D)) % D, b = v / ((size_t)D * D * D);
+        const size_t u00 = ((b * D2 + 2 * zz) * D2 + 2 * yy) * D2 + 2 * xx;
 #pragma unroll 2
-    for (int j = 0; j < NT; ++j) {
-        floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int j = 0; j < NT; ++j) {
+            floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int k = 0; k < KC; ++k) {
-            const bf16x8 bw = __builtin_bit_cast(bf16x8, *(const u32x4*)(wn + (size_t)(j * 16 + row) * CIN + k * 32 + grp * 8));
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw, av[k], acc, 0, 0, 0);
+            for (int k = 0; k < KC; ++k) {
+                const bf16x8 bw =
+                    __builtin_bit_cast(bf16x8, *(const u32x4*)(wn + (size_t)(j * 16 + row) * CIN + k * 32 + grp * 8));
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw, av[k], acc, 0, 0, 0);
+            }
+            const int sub = (j * 16) / COUT, co = (j * 16) % COUT + grp * 4;
+            const floatx4 bv = *(const floatx4*)(bias + co);
+            acc += bv;
+            const size_t u = u00 + ((sub >> 2) * D2 + ((sub >> 1) & 1)) * D2 + (sub & 1);
+            u32x2 q;
+            q[0] = f2bf2(acc[0], acc[1]);
+            q[1] = f2bf2(acc[2], acc[3]);
+            *(u32x2*)(y + u * COUT + co) = q;
         }
-        const int sub = (j * 16) / COUT, co = (j * 16) % COUT + grp * 4;
-        const floatx4 bv = *(const floatx4*)(bias + co);
-        acc += bv;
-        const size_t u = u00 + ((sub >> 2) * D2 + ((sub >> 1) & 1)) * D2 + (sub & 1);
-        u32x2 q;
-        q[0] = f2bf2(acc[0], acc[1]);
-        q[1] = f2bf2(acc[2], acc[3]);
-        *(u32x2*)(y + u * COUT + co) = q;
     }
 }
 
@@ -1676,8 +1683,8 @@ namespace {
 template <int CIN, int COUT>
 int up_fwd_t(const void* x, const void* wn, const float* b, void* y, int B, int D, hipStream_t st) {
     const size_t nseg = (size_t)B * D * D * D / 16;
-    k_up_fwd_mfma<CIN, COUT><<<(unsigned)((nseg + 3) / 4), 256, 0, st>>>((const bf16_t*)x, (const bf16_t*)wn, b,
-                                                                        (bf16_t*)y, B, D);
+    k_up_fwd_mfma<CIN, COUT><<<(unsigned)((nseg + 4 * kUpSegs - 1) / (4 * kUpSegs)), 256, 0, st>>>(
+        (const bf16_t*)x, (const bf16_t*)wn, b, (bf16_t*)y, B, D);
     CM21_LAUNCH_CHECK("k_up_fwd_mfma");
     return CM21_OK;
 }
