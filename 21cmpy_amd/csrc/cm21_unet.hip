@@ -22,15 +22,18 @@
 //   3^3 convolution, input gradient: the same kernel on the output gradient
 //     (masked by the ReLU of the forward output) with the weights transposed
 //     and spatially flipped.
-//   3^3 convolution, weight gradient: fp32 VALU over voxel slabs with the
-//     input halo tile and the masked output gradient in LDS; per-block
-//     partials reduced in a fixed order.
+//   3^3 convolution, weight gradient: v_mfma_f32_16x16x32_bf16 with K = 32
+//     voxels of an x-line, the masked output gradient and the input written
+//     to LDS channel-major (the input in three dx-shifted copies) so that
+//     every fragment is one aligned 16-byte read; fp32 VALU for boxes of
+//     side 16.  Per-block partials reduced in a fixed order.
 //   Input layer (density plus two constant conditioning channels, the
 //   latter folded into a per-sample bias): 3^3 taps on the fp32 matrix cores
-//   (v_mfma_f32_16x16x4f32).
-//   2x2x2 average pooling, 2x2x2 stride-2 transposed convolution (bf16
-//   matrix cores), the 1^3 head with sigmoid + binary cross entropy, and
-//   AdamW: fp32 VALU.
+//   (v_mfma_f32_16x16x4f32), forward and weight gradient.
+//   2x2x2 stride-2 transposed convolution: bf16 matrix cores forward and
+//   input gradient, fp32 matrix cores (K = 4 voxels) weight gradient.
+//   2x2x2 average pooling, the 1^3 head with sigmoid + binary cross entropy
+//   (fused into the last convolution for inference), and AdamW: fp32 VALU.
 #include <cmath>
 #include <cstring>
 
