@@ -8,10 +8,13 @@ barrier, mean collapsed fraction); the whole radius loop then runs on the GPU
 through ``cm21_find_bubbles`` with no host synchronisation.
 
 cosmolopy (``pb.sig_del``, ``pb.collapse_fraction``, find_HII_bubbles.py:57,60)
-is not available, so the barrier and sigma_min are given explicitly:
-``deltac`` (or ``growth`` = D(z), giving deltac = 1.686/D) and ``sigma_min2``.
-S(R) comes from ``sig0`` (a callable) or, by default, from the synthetic
-power-law spectrum of the box (cosmo_files.sigmas.PowerLawSpectrum).
+is not available.  The barrier and sigma_min are either given explicitly,
+``deltac`` (or ``growth`` = D(z), giving deltac = 1.686/D) and ``sigma_min2``,
+with S(R) from ``sig0`` (a callable) or by default from the synthetic
+power-law spectrum of the box (cosmo_files.sigmas.PowerLawSpectrum); or, with
+none of the three given, they follow from Z, INIT_PARAMS.ION_Tvir_MIN and the
+Planck13 cosmology of INIT_PARAMS (cosmo_files.planck: sig_del, D(z), and
+S(R) of the Planck13 spectrum), as the reference computes them.
 """
 import importlib
 import math
@@ -23,6 +26,7 @@ from scipy.special import erfc
 
 from .. import _lib
 from ..Parameter_files import INIT_PARAMS as P
+from ..cosmo_files.planck import barrier as planck_barrier
 from ..cosmo_files.sigmas import PowerLawSpectrum
 
 DELTA_C_LINEAR = 1.686
@@ -112,12 +116,22 @@ def conv_bubbles(deltax, param_dict, Z=None, scale=None, fil=1, update=0, LE=Fal
     L = float(box_len if box_len is not None else param_dict["BoxSize"])
     if np.shape(deltax) != (D, D, D):
         raise ValueError(f"deltax shape {np.shape(deltax)} does not match HIIdim={D}")
+    if deltac is None and growth is None and sigma_min2 is None:
+        # Planck-cosmology mode (find_HII_bubbles.py:57-60 without cosmolopy):
+        # the barrier from ION_Tvir_MIN at Z and S(R) of the Planck13 spectrum
+        z = Z if Z is not None else param_dict.get("Z")
+        if z is None:
+            raise ValueError("give Z (or param_dict['Z']) for the Planck mode, or deltac/growth and sigma_min2")
+        cos = planck_barrier(float(z))
+        deltac, sigma_min2 = cos["deltac"], cos["sigma_min2"]
+        if sig0 is None:
+            sig0 = cos["sig0"]
     if deltac is None:
         if growth is None:
-            raise ValueError("give deltac or growth (cosmolopy sig_del is not available)")
+            raise ValueError("give deltac or growth with sigma_min2, or neither (Planck mode)")
         deltac = DELTA_C_LINEAR / float(growth)
     if sigma_min2 is None:
-        raise ValueError("give sigma_min2 (cosmolopy sig_del is not available)")
+        raise ValueError("give sigma_min2 with deltac or growth, or neither (Planck mode)")
     zeta = float(P.ZETA if zeta is None else zeta)
     fgrowth = np.float32(float(deltac) / DELTA_C_LINEAR)
     sig0_fn = sig0 if sig0 is not None else _default_sig0(D, L)
@@ -195,11 +209,12 @@ def run(opts, args):
         Z = param_dict["z"]
         LE = False
     growth = getattr(opts, "GROWTH", None)
-    if growth is None:
-        raise ValueError("opts.GROWTH (linear growth D(z)) is required without cosmolopy")
+    smin2 = getattr(opts, "SIGMA_MIN2", None)
+    if (growth is None) != (smin2 is None):
+        raise ValueError("give both opts.GROWTH and opts.SIGMA_MIN2, or neither (Planck mode)")
+    extra = {} if growth is None else {"growth": float(growth), "sigma_min2": float(smin2)}
     ion = conv_bubbles(d1, param_dict, Z, fil=int(opts.FILTER_TYPE), update=int(opts.UPDATE_TYPE), LE=LE,
-                       quiet=bool(getattr(opts, "QUIET", False)), growth=float(growth),
-                       sigma_min2=float(opts.SIGMA_MIN2))
+                       quiet=bool(getattr(opts, "QUIET", False)), **extra)
     nf = 1 - float(np.mean(ion, dtype=np.float64))
     outname = xH_filename(Z, nf, opts.FILTER_TYPE, ion.shape[0], param_dict["BoxSize"])
     np.save(os.path.join(opts.DIR, outname), 1 - ion)
@@ -214,8 +229,9 @@ def main(argv=None):
     o.add_argument("-f", "--filt", dest="FILTER_TYPE", default=P.HII_FILTER, type=int)
     o.add_argument("-u", "--upd", dest="UPDATE_TYPE", default=P.FIND_BUBBLE_ALGORITHM, type=int)
     o.add_argument("-z", "--reds", dest="Z", default=12.0, type=float)
-    o.add_argument("-g", "--growth", dest="GROWTH", type=float, required=True)
-    o.add_argument("-s", "--sigma-min2", dest="SIGMA_MIN2", type=float, required=True)
+    o.add_argument("-g", "--growth", dest="GROWTH", type=float, default=None,
+                   help="linear growth D(z); with -s, instead of the Planck13 barrier")
+    o.add_argument("-s", "--sigma-min2", dest="SIGMA_MIN2", type=float, default=None)
     o.add_argument("-q", "--qui", dest="QUIET", action="store_true")
     opts = o.parse_args(argv)
     return run(opts, [opts.box])

@@ -7,8 +7,11 @@
   numpy.load (no pickles).
 * bubbles_D16.npz / bubbles_D32_nl.npz: seeded GRF boxes with the oracle's
   float32 ionized field (regression vectors for the oracle itself).
+* planck_tables.npz: every 10th row of cosmo_files/sig0.npz (mass in M_sun,
+  radius in Mpc, S(R)) for the Planck-cosmology mode (mass-radius relation and
+  S(R) of the Planck13 table).  Provenance as above.
 
-Run from the repository root: python tests/golden/make_golden.py
+Run from the repository root: python tests/golden/make_golden.py [planck]
 """
 import os
 import sys
@@ -60,7 +63,17 @@ def bubble_vectors():
                             box_len=L, sigma_min2=smin, deltac=deltac, zeta=30.0, LE=LE, fc_mean=fc_mean)
 
 
+def planck_tables():
+    s = np.load(os.path.join(REF, "sig0.npz"))
+    np.savez(os.path.join(HERE, "planck_tables.npz"), mass=s["mass"][::10], radius=s["radius"][::10],
+             sig0=s["sig0"][::10])
+
+
 if __name__ == "__main__":
-    tables()
-    bubble_vectors()
+    if sys.argv[1:] == ["planck"]:
+        planck_tables()
+    else:
+        tables()
+        bubble_vectors()
+        planck_tables()
     print("ok")

@@ -1,0 +1,43 @@
+"""conv_bubbles in the Planck-cosmology mode (no deltac / growth / sigma_min2
+given: the barrier from Z, INIT_PARAMS.ION_Tvir_MIN and ZETA, S(R) of the
+Planck13 spectrum) equals the call with the same scalars passed explicitly,
+and matches the FP32 oracle outside the threshold band."""
+import numpy as np
+import pytest
+from scipy.special import erfc
+
+from conftest import pkg
+from oracle import bubbles, grf, sigma
+
+pytestmark = pytest.mark.gpu
+
+
+def test_planck_mode_matches_explicit_and_oracle(gpu):
+    pl = pkg("cosmo_files.planck")
+    fb = pkg("Programs.find_HII_bubbles")
+    P = pkg("Parameter_files.INIT_PARAMS")
+    D, L, z = 64, 100.0, 9.0
+    delta = grf.c2r(grf.grf_half_spectrum(D, L, seed=21), D).astype(np.float32)
+    pd = {"BoxSize": L, "HIIdim": D, "Z": z}
+    print(f"planck mode D={D} z={z} zeta={P.ZETA}", flush=True)
+    got = fb.conv_bubbles(delta, pd, LE=True, quiet=True)
+    b = pl.barrier(z)
+    exp = fb.conv_bubbles(delta, pd, LE=True, quiet=True, deltac=b["deltac"], sigma_min2=b["sigma_min2"],
+                          sig0=b["sig0"])
+    assert np.array_equal(got, exp)
+    radii, _ = sigma.radius_schedule(L, D)
+    den = sigma.denominators(radii, b["sigma_min2"], b["sig0"])
+    fc = np.float32(erfc(b["deltac"] / np.sqrt(2 * b["sigma_min2"])))
+    gi = np.float32(b["deltac"] / 1.686)
+    want = bubbles.find_bubbles_f32(delta, L, radii, den, P.ZETA, b["deltac"], LE=True, growth_inv=gi, fc_mean=fc)
+    _, margin = bubbles.find_bubbles_f64(delta, L, radii, den, P.ZETA, b["deltac"], LE=True,
+                                         growth_inv=float(b["deltac"]) / 1.686, fc_mean=float(fc))
+    ok = margin > 1e-5
+    assert np.mean(ok) > 0.99
+    assert np.array_equal((got == 1)[ok], (want == 1)[ok])
+    part = ok & (want < 1) & (got < 1)
+    assert np.allclose(got[part], want[part], rtol=1e-4, atol=1e-7)
+    # a partly ionized box: flagged cells and painted partial values both occur
+    x_hi = 1.0 - float(got.mean())
+    print(f"x_HI {x_hi:.4f}, flagged {float((got == 1).mean()):.3f}", flush=True)
+    assert 0.02 < x_hi < 0.5 and 0.5 < float((got == 1).mean()) < 0.99
