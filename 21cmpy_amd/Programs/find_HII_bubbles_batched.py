@@ -23,6 +23,7 @@ import torch
 from scipy.special import erfc
 
 from .. import _lib
+from ..cosmo_files.planck import barrier as planck_barrier
 from ..cosmo_files.sigmas import PowerLawSpectrum
 from .find_HII_bubbles import DELTA_C_LINEAR, barrier_denominators, radius_schedule
 
@@ -92,7 +93,9 @@ class BatchedBubbles:
 
     specs: one dict per box with 'D', 'box_len', 'zeta', 'sigma_min2' and the
     barrier ('deltac', 'growth' or 'z'); optional 'sig0' (callable S(R)),
-    default the box's synthetic power-law spectrum.  Boxes are loaded with
+    default the box's synthetic power-law spectrum.  A spec with 'z' and none
+    of 'sigma_min2', 'deltac', 'growth' takes the Planck13 barrier and S(R)
+    (cosmo_files.planck), as conv_bubbles does.  Boxes are loaded with
     ``set_delta`` (or ``set_deltas``); ``run`` computes every ionized box;
     ``ionized(i)`` is box i's float32 (D,D,D) result (a view, overwritten by
     the next run).
@@ -103,6 +106,14 @@ class BatchedBubbles:
         if fil not in (1, 2):
             raise ValueError("the batched loop supports HII_FILTER 1 and 2")
         self.specs = [dict(s) for s in specs]
+        for s in self.specs:
+            if "sigma_min2" not in s and "deltac" not in s and "growth" not in s:
+                # Planck-cosmology mode: the barrier and S(R) from the box's z
+                if "z" not in s:
+                    raise ValueError("a spec needs 'sigma_min2' with 'deltac', 'growth' or 'z', or 'z' alone")
+                b = planck_barrier(float(s["z"]))
+                s["deltac"], s["sigma_min2"] = b["deltac"], b["sigma_min2"]
+                s.setdefault("sig0", b["sig0"])
         self.LE = bool(LE)
         self.fil = int(fil)
         self.stream = stream

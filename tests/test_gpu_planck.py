@@ -41,3 +41,24 @@ def test_planck_mode_matches_explicit_and_oracle(gpu):
     x_hi = 1.0 - float(got.mean())
     print(f"x_HI {x_hi:.4f}, flagged {float((got == 1).mean()):.3f}", flush=True)
     assert 0.02 < x_hi < 0.5 and 0.5 < float((got == 1).mean()) < 0.99
+
+
+def test_batched_planck_mode_matches_single_box(gpu):
+    """Boxes of a batch given only their z take the Planck13 barrier and S(R)
+    and agree with conv_bubbles' Planck mode box by box."""
+    fb = pkg("Programs.find_HII_bubbles")
+    bm = pkg("Programs.find_HII_bubbles_batched")
+    D, L = 64, 100.0
+    zs = [9.0, 9.5]
+    deltas = [grf.c2r(grf.grf_half_spectrum(D, L, seed=30 + i), D).astype(np.float32) for i in range(len(zs))]
+    specs = [{"D": D, "box_len": L, "zeta": 40.0, "z": z} for z in zs]
+    bb = bm.BatchedBubbles(specs, fil=1, LE=True, graph=False)
+    bb.set_deltas(deltas)
+    bb.run()
+    for i, z in enumerate(zs):
+        got = bb.ionized(i).cpu().numpy()
+        single = fb.conv_bubbles(deltas[i], {"BoxSize": L, "HIIdim": D, "Z": z}, LE=True, quiet=True, zeta=40.0)
+        print(f"box {i} z={z}: x_HI {1 - got.mean():.4f}", flush=True)
+        assert np.mean((got == 1) != (single == 1)) <= 1e-5
+        both = (got < 1) & (single < 1)
+        assert np.allclose(got[both], single[both], rtol=0, atol=1e-5)
