@@ -16,6 +16,8 @@ import torch
 
 from .. import gpu, _lib
 from ..Parameter_files import INIT_PARAMS as P
+from ..cosmo_files import planck
+from . import initialize
 
 
 def displacement_scale(growth, growth_init, box_len):
@@ -50,6 +52,45 @@ def evolve_linear(z, deltax, growth, box_len=None, save=False, out_dir=None):
         os.makedirs(d, exist_ok=True)
         np.save(os.path.join(d, updated_name(z, upd.shape[0], L)), upd.cpu().numpy())
     return upd
+
+
+def velocity_name(axis, HII_DIM, BOX_LEN):
+    return "updated_v{0}overddot_{1:d}_{2:.0f}Mpc".format(axis, int(HII_DIM), float(BOX_LEN))
+
+
+def velocities_at(z, deltax, HII_DIM=None, box_len=None, ddt_over_d=None, save=False, out_dir=None):
+    """Velocity boxes at z from the z = 0 linear density (perturb_field.py:
+    111-126): v = (dD/dt)/D(z) * psi, psi~ = i k / k^2 delta~, top-hat filtered
+    at L_FACTOR * BOX_LEN / HII_DIM, inverse transformed and subsampled to
+    HII_DIM (the initialize velocity path with the growth-rate factor).
+    deltax: DIM^3 CUDA tensor.  ddt_over_d: (dD/dt)/D in s^-1, default the
+    Planck13 value (cosmo_files.planck).  Returns (vx, vy, vz) in the array-axis
+    order of initialize.init's velocities."""
+    _lib.require_gpu()
+    L = float(P.BOX_LEN if box_len is None else box_len)
+    HII = int(P.HII_DIM if HII_DIM is None else HII_DIM)
+    DIM = int(deltax.shape[0])
+    if DIM % HII:
+        raise ValueError(f"HII_DIM={HII} must divide DIM={DIM}")
+    f = planck.ddt_growth_over_growth(float(z)) if ddt_over_d is None else float(ddt_over_d)
+    spec = gpu.r2c(deltax)  # DIM^3 times the c2r-ready spectrum
+    R = initialize.smoothing_radius(L, HII)
+    work = torch.empty_like(spec)
+    large = torch.empty((DIM, DIM, DIM), dtype=torch.float32, device=deltax.device)
+    scale = np.float32(f / float(DIM) ** 3)
+    vs = []
+    for comp in range(3):
+        gpu.velocity(spec, L, comp, filter_type=0, R=R, out=work)
+        gpu.c2r(work, out=large, preserve_input=False)
+        v = gpu.subsample(large, HII)
+        v.mul_(scale)
+        vs.append(v)
+    if save:
+        d = os.path.join(out_dir or P.IO_DIR, "Boxes")
+        os.makedirs(d, exist_ok=True)
+        for axis, v in zip("xyz", vs):
+            np.save(os.path.join(d, velocity_name(axis, HII, L)), v.cpu().numpy())
+    return tuple(vs)
 
 
 def run(z, growth, growth_init, DIM=None, HII_DIM=None, BOX_LEN=None, scheme="cic", linear=None, io_dir=None):

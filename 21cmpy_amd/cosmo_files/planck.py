@@ -19,7 +19,8 @@ definitions, for COSMO = PLANCK13 (Parameter_files/INIT_PARAMS.py):
 * D(z): Carroll, Press & Turner (1992, ARA&A 30, 499) fit for a flat
   Lambda universe, normalised to D(0) = 1 (the approximation cosmolopy's
   fgrowth documents), and the Liddle et al. fit the reference's own C code
-  carries (``dicke``, Programs/ps.c:16-45).
+  carries (``dicke``, Programs/ps.c:16-45); (dD/dt)/D for the velocities
+  at z (Programs/cosmo_functions.py:7-21).
 * M_vir(T_vir, z): Barkana & Loeb (2001, Phys. Rep. 349, 125) eq. 26 with
   Delta_c of Bryan & Norman (1998) for a flat universe; mean molecular
   weight 0.59 at T >= 1e4 K (ionized primordial gas), 1.22 below.
@@ -167,6 +168,26 @@ def dicke(z, cosmo=None):
     return float(out) if out.ndim == 0 else out
 
 
+H0_PER_H = 1.0e5 / 3.0856775814913673e22  # 100 km/s/Mpc in s^-1
+
+
+def hubble(z, cosmo=None):
+    """H(z) in s^-1."""
+    c = cosmo_params(cosmo)
+    om, ol = c["omega_M_0"], c["omega_lambda_0"]
+    e2 = om * (1.0 + z) ** 3 + ol + (1.0 - om - ol) * (1.0 + z) ** 2
+    return c["h"] * H0_PER_H * math.sqrt(e2)
+
+
+def ddt_growth_over_growth(z, cosmo=None):
+    """(dD/dt) / D at z in s^-1 (cosmo_functions.py:7-21 dDdtoverD): dD/dz by a
+    central difference, dt/dz = -1 / ((1+z) H(z))."""
+    dz = 1.0e-4 * max(1.0, float(z))
+    lo = max(float(z) - dz, 0.0)
+    ddz = (fgrowth(float(z) + dz, cosmo) - fgrowth(lo, cosmo)) / (float(z) + dz - lo)
+    return -ddz * (1.0 + z) * hubble(z, cosmo) / fgrowth(z, cosmo)
+
+
 # ------------------------------------------------------------------ haloes
 
 def virial_mass(t_vir, z, cosmo=None, mu=None):
@@ -222,5 +243,5 @@ def sigma_8(spectrum=None, cosmo=None):
 
 
 __all__ = ["TableSpectrum", "EHSpectrum", "eh_nowiggle_transfer", "planck_spectrum", "sigma_r", "fgrowth", "dicke",
-           "virial_mass", "mass_to_radius", "sig_del", "collapse_fraction", "barrier", "sigma_8",
-           "tophat_window_closed"]
+           "hubble", "ddt_growth_over_growth", "virial_mass", "mass_to_radius", "sig_del", "collapse_fraction",
+           "barrier", "sigma_8"]

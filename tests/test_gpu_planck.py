@@ -62,3 +62,22 @@ def test_batched_planck_mode_matches_single_box(gpu):
         assert np.mean((got == 1) != (single == 1)) <= 1e-5
         both = (got < 1) & (single < 1)
         assert np.allclose(got[both], single[both], rtol=0, atol=1e-5)
+
+
+def test_velocities_at_z_are_growth_rate_times_initial(gpu):
+    """perturb_field.velocities_at (perturb_field.py:111-126) from the z = 0
+    density equals (dD/dt)/D times initialize's velocity boxes (the same
+    filtered, subsampled psi), box for box."""
+    init = pkg("Programs.initialize")
+    pf = pkg("Programs.perturb_field")
+    pl = pkg("cosmo_files.planck")
+    DIM, HII, L, z = 64, 32, 50.0, 9.0
+    ics = init.init(DIM=DIM, HII_DIM=HII, BOX_LEN=L, seed=5, save=False)
+    f = pl.ddt_growth_over_growth(z)
+    vs = pf.velocities_at(z, ics["deltax"], HII_DIM=HII, box_len=L)
+    for axis, (v, v0) in enumerate(zip(vs, ics["v"])):
+        ref = (v0.double() * f).cpu()
+        err = float((v.double().cpu() - ref).abs().max())
+        scale = float(ref.abs().max())
+        print(f"axis {axis}: max err {err:.3g} of {scale:.3g}", flush=True)
+        assert err <= 1e-4 * scale

@@ -102,3 +102,16 @@ def test_planck_mode_host_setup(D, L, z):
     radii, denoms, fc_mean = fb.bubble_setup(D, L, None, b["deltac"], b["sigma_min2"], b["sig0"])
     assert len(radii) > 10 and np.all(np.isfinite(denoms)) and np.all(denoms > 0)
     assert fc_mean == pytest.approx(np.float32(b["fc_mean"]), rel=1e-6)
+
+
+def test_growth_rate_limits():
+    """(dD/dt)/D = f(z) H(z): f -> 1 in matter domination; today f close to
+    Omega_m^0.55 (the CPT fit's derivative, as the reference's finite
+    difference of fgrowth gives it, cosmo_functions.py:7-21)."""
+    pl = pkg("cosmo_files.planck")
+    c = pl.cosmo_params()
+    assert pl.ddt_growth_over_growth(1000.0) / pl.hubble(1000.0) == pytest.approx(1.0, abs=1e-5)
+    assert pl.ddt_growth_over_growth(8.0) / pl.hubble(8.0) == pytest.approx(1.0, abs=3e-3)
+    assert pl.ddt_growth_over_growth(0.0) / pl.hubble(0.0) == pytest.approx(c["omega_M_0"] ** 0.55, rel=0.05)
+    # H0 = 100 h km/s/Mpc
+    assert pl.hubble(0.0) == pytest.approx(c["h"] * 100.0 / 3.0856775814913673e19, rel=1e-12)
