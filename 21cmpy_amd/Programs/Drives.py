@@ -8,26 +8,30 @@ computed by ONE batched find_HII_bubbles call (the redshifts share HII_DIM
 and BOX_LEN, so they form one group with their own barriers 1.686/D(z)),
 followed by the brightness temperature and P21(k) of each redshift.
 
-Growth factors come from a callable (default D(z) = 0.12 * 9 / (1 + z),
-anchored at D(8) = 0.12) because cosmolopy is not available; sigma_min^2 and
-zeta are explicit.
+Without ``sigma_min2`` the run follows the Planck13 cosmology of INIT_PARAMS
+as the reference does with cosmolopy: D(z) and, per redshift, the barrier and
+S(R) from cosmo_files.planck (ION_Tvir_MIN).  With an explicit ``sigma_min2``
+the growth factors come from ``growth`` (a callable, default D(z) = 0.12 * 9 /
+(1 + z), anchored at D(8) = 0.12) and every redshift uses that sigma_min^2.
 """
 import numpy as np
 
 from ..Parameter_files import INIT_PARAMS as P
+from ..cosmo_files import planck
 from . import delta_T, initialize, perturb_field
 from .find_HII_bubbles_batched import BatchedBubbles, linear_growth
 
 
 def drive_zscroll_noTs(ZHIGH=16.0, ZLOW=10.0, DZ=-2.0, DIM=None, HII_DIM=None, BOX_LEN=None, seed=0, growth=None,
-                       zeta=None, sigma_min2=4.0, scheme="cic", bubbles=True, nbins=40, save=False, out_dir=None,
+                       zeta=None, sigma_min2=None, scheme="cic", bubbles=True, nbins=40, save=False, out_dir=None,
                        quiet=False):
     """Returns {'z', 'density': [HII_DIM^3 tensors]} and, with bubbles,
     'ionized' (tensors), 'x_HI' and 'P21' (per z: (k_edges, k_ave, ps_ave))."""
     DIM = int(P.DIM if DIM is None else DIM)
     HII_DIM = int(P.HII_DIM if HII_DIM is None else HII_DIM)
     L = float(P.BOX_LEN if BOX_LEN is None else BOX_LEN)
-    grow = growth if growth is not None else linear_growth
+    planck_mode = sigma_min2 is None
+    grow = growth if growth is not None else (planck.fgrowth if planck_mode else linear_growth)
     zeta = float(P.ZETA if zeta is None else zeta)
     if not quiet:
         print("Now calling initialize")
@@ -43,8 +47,11 @@ def drive_zscroll_noTs(ZHIGH=16.0, ZLOW=10.0, DZ=-2.0, DIM=None, HII_DIM=None, B
     out = {"z": zs, "density": dens}
     if not bubbles:
         return out
-    specs = [{"D": HII_DIM, "box_len": L, "zeta": zeta, "deltac": 1.686 / grow(z), "sigma_min2": sigma_min2}
-             for z in zs]
+    if planck_mode:
+        specs = [{"D": HII_DIM, "box_len": L, "zeta": zeta, "z": float(z)} for z in zs]
+    else:
+        specs = [{"D": HII_DIM, "box_len": L, "zeta": zeta, "deltac": 1.686 / grow(z), "sigma_min2": sigma_min2}
+                 for z in zs]
     bb = BatchedBubbles(specs, fil=P.HII_FILTER, LE=False, graph=False)
     bb.set_deltas(dens)
     bb.run()

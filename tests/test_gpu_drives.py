@@ -38,3 +38,24 @@ def test_zscroll_matches_stage_by_stage(gpu):
         assert len(ps) == 40 and np.all(np.isfinite(ps))
     # later redshifts are more ionized
     assert out["x_HI"][0] >= out["x_HI"][1] >= out["x_HI"][2]
+
+
+def test_zscroll_planck_mode(gpu):
+    """Without sigma_min2 the scroll follows the Planck13 cosmology (D(z) and the
+    per-redshift barrier of cosmo_files.planck); each redshift's box equals
+    conv_bubbles' Planck mode on the same density."""
+    drv = pkg("Programs.Drives")
+    fb = pkg("Programs.find_HII_bubbles")
+    DIM, D, L = 128, 64, 100.0
+    out = drv.drive_zscroll_noTs(12.0, 9.5, -1.0, DIM=DIM, HII_DIM=D, BOX_LEN=L, seed=4, quiet=True)
+    assert list(out["z"]) == [12.0, 11.0, 10.0]
+    for i, z in enumerate(out["z"]):
+        single = fb.conv_bubbles(out["density"][i], {"BoxSize": L, "HIIdim": D, "Z": float(z)}, LE=False,
+                                 quiet=True).cpu().numpy()
+        got = out["ionized"][i].cpu().numpy()
+        print(f"planck z={z}: x_HI {out['x_HI'][i]:.4f}", flush=True)
+        assert np.mean((got == 1) != (single == 1)) <= 1e-5
+        both = (got < 1) & (single < 1)
+        assert np.allclose(got[both], single[both], rtol=0, atol=1e-5)
+    assert out["x_HI"][0] > out["x_HI"][1] > out["x_HI"][2]
+    assert 0.0 < out["x_HI"][2] and out["x_HI"][0] < 1.0
