@@ -86,6 +86,7 @@ class UnetConv(ctypes.Structure):
         ("bias", ctypes.POINTER(ctypes.c_float)), ("bias_per_sample", ctypes.c_int), ("relu", ctypes.c_int),
         ("y0", ctypes.c_void_p), ("y1", ctypes.c_void_p), ("algo", ctypes.c_int),
         ("head", ctypes.POINTER(ctypes.c_float)), ("prob", ctypes.POINTER(ctypes.c_float)),
+        ("pool", ctypes.c_void_p),
     ]
 
 

@@ -93,6 +93,7 @@ struct ConvArgs {
     bf16_t* y1;
     const float* head;  // fused 1^3 head (cout = 16): w[16], b; or NULL
     float* prob;        // its sigmoid output (B,D,D,D) fp32
+    bf16_t* pool;       // 2x2x2 mean of y0 (B,D/2,D/2,D/2,cout), written by the POOL form
 };
 
 // The MFMA takes the weights as its A operand and the activations as its B
@@ -439,8 +440,14 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
 // most 14 weight fragments (K slices x N tiles) keep every B fragment in
 // registers for the whole walk; the others stream them per K slice,
 // double-buffered.
-template <int CIN, int COUT, int TY, int NZ, bool HEAD = false>
-__global__ __launch_bounds__(256) void k_conv3_slide(ConvArgs a) {
+//
+// POOL (16 -> 16 channels, two lines per wave): the 2x2x2 mean of the stored
+// output goes to a.pool as well.  Lanes of rows 2i and 2i+1 hold an x-pair,
+// a wave's two lines the y-pair, consecutive planes the z-pair; the sum runs
+// in the order of k_pool_fwd (z, y, x, from 0), so the pooled box equals that
+// kernel's output bit for bit.
+template <int CIN, int COUT, int TY, int NZ, bool HEAD = false, bool POOL = false>
+__global__ __launch_bounds__(256, POOL ? 4 : 1) void k_conv3_slide(ConvArgs a) {
     constexpr int HY = TY + 2, HX = 18, CP = CIN >= 32 ? CIN + 8 : CIN;
     constexpr int PLANE = HY * HX * CP;               // bf16 per ring slot
     constexpr int MW = TY / 4, NT = COUT / 16;
@@ -520,6 +527,9 @@ __global__ __launch_bounds__(256) void k_conv3_slide(ConvArgs a) {
     }
     floatx4 bv[NT];
     load_bias<NT>(a, b, grp, bv);
+    static_assert(!POOL || (COUT == 16 && MW == 2 && NZ % 2 == 0 && !HEAD), "pool pairs");
+    // pool sums of channels 4 grp .. +3, carried from an even plane to the next
+    __shared__ __attribute__((aligned(16))) float psum[POOL ? 256 * 4 : 4];
     __syncthreads();
     for (int oz = 0; oz < NZ; ++oz) {
         u32x4 pf[NPF];
@@ -591,6 +601,37 @@ __global__ __launch_bounds__(256) void k_conv3_slide(ConvArgs a) {
             }
         }
         if (more) store(oz + 3, pf);
+        // (after the ring store, so that the prefetch registers are free)
+        if constexpr (POOL) {
+            // per channel: the stored (bf16) values of lines 0 and 1 and of
+            // the x-neighbour row, added to the sums of the even plane (kept
+            // in the lane's own LDS slot, not in registers)
+            const bool odd_plane = oz & 1;
+            floatx4 o;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float r0 = acc[0][0][i] + bv[0][i], r1 = acc[1][0][i] + bv[0][i];
+                const float s0 = __uint_as_float((unsigned)f2bf(a.relu ? fmaxf(r0, 0.0f) : r0) << 16);
+                const float s1 = __uint_as_float((unsigned)f2bf(a.relu ? fmaxf(r1, 0.0f) : r1) << 16);
+                const float n0 = __shfl_xor(s0, 1, 64), n1 = __shfl_xor(s1, 1, 64);
+                o[i] = (((odd_plane ? psum[t * 4 + i] : 0.0f) + s0) + n0) + s1 + n1;
+            }
+            if (!odd_plane) {
+                *(floatx4*)(psum + t * 4) = o;
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) o[i] *= 0.125f;
+                if ((row & 1) == 0) {
+                    const int Dh = D / 2;
+                    const size_t pv =
+                        (((size_t)b * Dh + (z0 + oz) / 2) * Dh + (y0 + wave * MW) / 2) * Dh + (x0 + row) / 2;
+                    u32x2 q;
+                    q[0] = f2bf2(o[0], o[1]);
+                    q[1] = f2bf2(o[2], o[3]);
+                    *(u32x2*)(a.pool + pv * 16 + grp * 4) = q;
+                }
+            }
+        }
         __syncthreads();
     }
 }
@@ -978,42 +1019,70 @@ __global__ __launch_bounds__(256) void k_up_fwd_mfma(const bf16_t* __restrict__ 
     // weights are the A operand (N = s * COUT + co), voxels the B operand:
     // lane (row, grp) ends with outputs n = 16 j + 4 grp .. +3 of input voxel
     // v0 + row, i.e. 4 consecutive channels of one output voxel (16 j lies
-    // inside one sub-position s because COUT is a multiple of 16)
-    constexpr int KC = CIN / 32, NT = 8 * COUT / 16;
+    // inside one sub-position s because COUT is a multiple of 16).  The tiles
+    // of sub-positions s (dx = 0) and s + 1 (dx = 1) are formed together and
+    // lane groups 2g and 2g + 1 swap halves, so that each lane stores 8
+    // consecutive channels of one output voxel (16 bytes) and the four lanes
+    // of a row cover both output voxels of the x-pair.  Voxel indices are
+    // 32-bit (the host checks B D^3 < 2^32).
+    constexpr int KC = CIN / 32, TPS = COUT / 16;  // N tiles per sub-position
+    constexpr bool KEEPW = 8 * TPS * KC <= 8;      // every weight fragment stays in registers
     static_assert(CIN % 32 == 0 && COUT % 16 == 0, "channel counts");
     const int lane = threadIdx.x & 63, row = lane & 15, grp = lane >> 4;
-    const size_t nseg = (size_t)B * D * D * D / 16;
-    const size_t seg0 = ((size_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * kUpSegs;
+    const bool odd = grp & 1;
+    const unsigned nseg = (unsigned)B * D * D * D / 16;
+    const unsigned seg0 = (blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * kUpSegs;
+    auto wfrag = [&](int j, int k) {
+        return __builtin_bit_cast(bf16x8, *(const u32x4*)(wn + (j * 16 + row) * CIN + k * 32 + grp * 8));
+    };
+    bf16x8 wk[KEEPW ? 8 * TPS * KC : 1];
+    if constexpr (KEEPW) {
+#pragma unroll
+        for (int j = 0; j < 8 * TPS; ++j)
+#pragma unroll
+            for (int k = 0; k < KC; ++k) wk[j * KC + k] = wfrag(j, k);
+    }
+    const unsigned ud = (unsigned)D;
+    const size_t D2 = 2 * (size_t)D;
     for (int sg = 0; sg < kUpSegs; ++sg) {
-        const size_t seg = seg0 + sg;
+        const unsigned seg = seg0 + sg;
         if (seg >= nseg) return;  // whole wave
-        const size_t v0 = seg * 16;
+        const unsigned v = seg * 16 + row;  // this lane's input voxel
         bf16x8 av[KC];
 #pragma unroll
         for (int k = 0; k < KC; ++k)
-            av[k] = __builtin_bit_cast(bf16x8, *(const u32x4*)(x + (v0 + row) * CIN + k * 32 + grp * 8));
-        const size_t D2 = 2 * (size_t)D;
-        // output voxel of this lane's input voxel at sub-position 0
-        const size_t v = v0 + row;
-        const size_t xx = v % D, yy = (v / D) % D, zz = (v / ((size_t)D * D)) % D, b = v / ((size_t)D * D * D);
-        const size_t u00 = ((b * D2 + 2 * zz) * D2 + 2 * yy) * D2 + 2 * xx;
-#pragma unroll 2
-        for (int j = 0; j < NT; ++j) {
-            floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            av[k] = __builtin_bit_cast(bf16x8, *(const u32x4*)(x + (size_t)v * CIN + k * 32 + grp * 8));
+        const unsigned xx = v % ud, r1 = v / ud, yy = r1 % ud, r2 = r1 / ud, zz = r2 % ud, b = r2 / ud;
+        // output voxel of the lane's input voxel at sub-position (0, 0, dx = odd)
+        const size_t u0 = (((size_t)b * D2 + 2 * zz) * D2 + 2 * yy) * D2 + 2 * xx + (odd ? 1 : 0);
 #pragma unroll
-            for (int k = 0; k < KC; ++k) {
-                const bf16x8 bw =
-                    __builtin_bit_cast(bf16x8, *(const u32x4*)(wn + (size_t)(j * 16 + row) * CIN + k * 32 + grp * 8));
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw, av[k], acc, 0, 0, 0);
+        for (int s = 0; s < 8; s += 2) {
+#pragma unroll
+            for (int h = 0; h < TPS; ++h) {
+                const int j0 = s * TPS + h, j1 = (s + 1) * TPS + h;
+                floatx4 a0 = {0.0f, 0.0f, 0.0f, 0.0f}, a1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int k = 0; k < KC; ++k) {
+                    a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(KEEPW ? wk[j0 * KC + k] : wfrag(j0, k), av[k], a0,
+                                                                 0, 0, 0);
+                    a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(KEEPW ? wk[j1 * KC + k] : wfrag(j1, k), av[k], a1,
+                                                                 0, 0, 0);
+                }
+                const floatx4 bv = *(const floatx4*)(bias + h * 16 + grp * 4);
+                a0 += bv;
+                a1 += bv;
+                const unsigned p0 = f2bf2(a0[0], a0[1]), p1 = f2bf2(a0[2], a0[3]);
+                const unsigned q0 = f2bf2(a1[0], a1[1]), q1 = f2bf2(a1[2], a1[3]);
+                // even lane groups pass on their dx = 1 channels, odd ones their dx = 0 channels
+                const unsigned r0 = __shfl_xor(odd ? p0 : q0, 16, 64), r1 = __shfl_xor(odd ? p1 : q1, 16, 64);
+                u32x4 o;
+                o[0] = odd ? r0 : p0;
+                o[1] = odd ? r1 : p1;
+                o[2] = odd ? q0 : r0;
+                o[3] = odd ? q1 : r1;
+                const size_t u = u0 + ((s >> 2) * D2 + ((s >> 1) & 1)) * D2;
+                *(u32x4*)(y + u * COUT + h * 16 + (grp & 2) * 4) = o;
             }
-            const int sub = (j * 16) / COUT, co = (j * 16) % COUT + grp * 4;
-            const floatx4 bv = *(const floatx4*)(bias + co);
-            acc += bv;
-            const size_t u = u00 + ((sub >> 2) * D2 + ((sub >> 1) & 1)) * D2 + (sub & 1);
-            u32x2 q;
-            q[0] = f2bf2(acc[0], acc[1]);
-            q[1] = f2bf2(acc[2], acc[3]);
-            *(u32x2*)(y + u * COUT + co) = q;
         }
     }
 }
@@ -1242,9 +1311,21 @@ __global__ __launch_bounds__(256) void k_in_fwd(const float* __restrict__ delta,
     const int y0 = (blockIdx.y % nyt) * TY, z0 = (blockIdx.y / nyt) * TZ;
     const int b = blockIdx.z;
     const size_t vb = (size_t)b * D * D * D;
-    for (int i = threadIdx.x; i < HZ * HY * HX; i += 256) {
-        const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
-        tile[i] = delta[vb + ((size_t)wrap(z0 + hz - 1, D) * D + wrap(y0 + hy - 1, D)) * D + wrap(x0 + hx - 1, D)];
+    // every halo load of the thread in flight before the LDS writes
+    constexpr int NH = HZ * HY * HX, NIT = (NH + 255) / 256;
+    float hv[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int i = threadIdx.x + it * 256;
+        if (i < NH) {
+            const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
+            hv[it] = delta[vb + ((size_t)wrap(z0 + hz - 1, D) * D + wrap(y0 + hy - 1, D)) * D + wrap(x0 + hx - 1, D)];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int i = threadIdx.x + it * 256;
+        if (i < NH) tile[i] = hv[it];
     }
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1483,10 +1564,15 @@ void launch_conv(const ConvArgs& a, hipStream_t st) {
     const dim3 grid(a.D / 16, (a.D / TZ) * (a.D / TY), a.B);
     k_conv3_mfma<CIN, COUT, TZ, TY, CH><<<grid, 256, 0, st>>>(a);
 }
-template <int CIN, int COUT, int TY, int NZ, bool HEAD = false>
+template <int CIN, int COUT, int TY, int NZ, bool HEAD = false, bool POOL = false>
 void launch_slide(const ConvArgs& a, hipStream_t st) {
     const dim3 grid(a.D / 16, (a.D / TY) * (a.D / NZ), a.B);
-    k_conv3_slide<CIN, COUT, TY, NZ, HEAD><<<grid, 256, 0, st>>>(a);
+    k_conv3_slide<CIN, COUT, TY, NZ, HEAD, POOL><<<grid, 256, 0, st>>>(a);
+}
+// sliding form with the pool of its output fused (16 -> 16 channels, D >= 64)
+ConvFn slide_pool(int cin, int cout, int D) {
+    if (cin != 16 || cout != 16 || D < 64) return nullptr;
+    return D >= 128 ? launch_slide<16, 16, 8, 16, false, true> : launch_slide<16, 16, 8, 8, false, true>;
 }
 
 // Tiled form: 4 x 4 x 16 (cin <= 32) or 4 x 2 x 16 (cin = 64) voxels per
@@ -1562,6 +1648,14 @@ constexpr int kInBlocks = 1024;     // line blocks of the input-layer weight gra
 
 bool valid_unet_dim(int D) { return D >= 16 && D <= 1024 && D % 16 == 0; }
 
+// 2x2x2 mean pool, C a multiple of 8, B * D / 2 < 65536 (grid z)
+int pool_fwd_launch(const bf16_t* x, bf16_t* y, int B, int D, int C, hipStream_t st) {
+    const int Dh = D / 2;
+    k_pool_fwd<<<dim3((Dh * (C / 8) + 63) / 64, Dh, B * Dh), 64, 0, st>>>(x, y, D, C);
+    CM21_LAUNCH_CHECK("k_pool_fwd");
+    return CM21_OK;
+}
+
 }  // namespace
 }  // namespace cm21
 
@@ -1580,6 +1674,8 @@ extern "C" int cm21_unet_conv3_fwd(const cm21_unet_conv* c, void* stream) {
     CM21_REQUIRE(c->c0 == c->cin || c->x1, "cm21_unet_conv3_fwd: x1 is NULL with c0 < cin");
     CM21_REQUIRE(c->o0 == c->cout || c->y1, "cm21_unet_conv3_fwd: y1 is NULL with o0 < cout");
     CM21_REQUIRE(!c->mask || c->c0 == c->cin, "cm21_unet_conv3_fwd: mask needs a single input");
+    CM21_REQUIRE(!c->pool || (!c->head && c->o0 == c->cout && (size_t)c->B * (c->D / 2) < 65536),
+                 "cm21_unet_conv3_fwd: pool needs every output channel in y0 and no head");
     ConvArgs a{};
     a.B = c->B;
     a.D = c->D;
@@ -1612,8 +1708,16 @@ extern "C" int cm21_unet_conv3_fwd(const cm21_unet_conv* c, void* stream) {
                           : (c->D < 64 ? s->tile_small
                                        : (algo == 1 ? s->tile_big : (c->D >= 128 ? s->slide128 : s->slide64)));
     if (c->head && algo == 2) fn = c->D >= 128 ? s->slide128_head : s->slide64_head;
+    // the pool is fused into the sliding form of 16 -> 16 channels; the other
+    // forms pool the stored output afterwards (the same sums, bit for bit)
+    const ConvFn fused_pool = c->pool && algo == 2 ? slide_pool(c->cin, c->cout, c->D) : nullptr;
+    if (fused_pool) {
+        a.pool = (bf16_t*)c->pool;
+        fn = fused_pool;
+    }
     fn(a, st);
     CM21_LAUNCH_CHECK("k_conv3_mfma");
+    if (c->pool && !fused_pool) return pool_fwd_launch(a.y0, (bf16_t*)c->pool, c->B, c->D, c->cout, st);
     return CM21_OK;
 }
 
@@ -1666,11 +1770,7 @@ extern "C" int cm21_unet_conv3_wgrad(const cm21_unet_wgrad* w, void* stream) {
 extern "C" int cm21_unet_pool_fwd(const void* x, void* y, int B, int D, int C, void* stream) {
     CM21_REQUIRE(x && y && B >= 1 && D >= 2 && D % 2 == 0 && C >= 8 && C % 8 == 0 && (size_t)B * (D / 2) < 65536,
                  "cm21_unet_pool_fwd: bad arguments");
-    const int Dh = D / 2;
-    k_pool_fwd<<<dim3((Dh * (C / 8) + 63) / 64, Dh, B * Dh), 64, 0, (hipStream_t)stream>>>((const bf16_t*)x,
-                                                                                          (bf16_t*)y, D, C);
-    CM21_LAUNCH_CHECK("k_pool_fwd");
-    return CM21_OK;
+    return pool_fwd_launch((const bf16_t*)x, (bf16_t*)y, B, D, C, (hipStream_t)stream);
 }
 
 extern "C" int cm21_unet_pool_bwd(const void* dy, void* dx, int B, int D, int C, int accumulate, void* stream) {
@@ -1726,7 +1826,8 @@ int up_bwd_t(const void* x, const void* dy, const void* wk, void* dx, float* dw,
 
 extern "C" int cm21_unet_up_fwd(const void* x, const void* wn, const float* bias, void* y, int B, int D, int cin,
                                 int cout, void* stream) {
-    CM21_REQUIRE(x && wn && bias && y && B >= 1 && D >= 2 && ((size_t)B * D * D * D) % 16 == 0,
+    CM21_REQUIRE(x && wn && bias && y && B >= 1 && D >= 2 && ((size_t)B * D * D * D) % 16 == 0 &&
+                     (size_t)B * D * D * D < ((size_t)1 << 32),
                  "cm21_unet_up_fwd: bad arguments");
     hipStream_t st = (hipStream_t)stream;
     if (cin == 64 && cout == 32) return up_fwd_t<64, 32>(x, wn, bias, y, B, D, st);

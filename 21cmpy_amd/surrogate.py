@@ -194,10 +194,11 @@ class UNet3D:
 
     # -------------------------------------------------------------- layers
     def _conv(self, x0, w, y0, cin, cout, D, x1=None, mask=None, bias=None, relu=True, y1=None, c0=None, o0=None,
-              bias_per_sample=False, head=None, prob=None):
+              bias_per_sample=False, head=None, prob=None, pool=None):
         c = _lib.UnetConv(B=self.B, D=D, cin=cin, cout=cout, c0=c0 or cin, o0=o0 or cout, x0=_vp(x0), x1=_vp(x1),
                           mask=_vp(mask), w=_vp(w), bias=_fp(bias), bias_per_sample=1 if bias_per_sample else 0,
-                          relu=1 if relu else 0, y0=_vp(y0), y1=_vp(y1), head=_fp(head), prob=_fp(prob))
+                          relu=1 if relu else 0, y0=_vp(y0), y1=_vp(y1), head=_fp(head), prob=_fp(prob),
+                          pool=_vp(pool))
         _lib.call("cm21_unet_conv3_fwd", ctypes.byref(c), self._st())
 
     def _wgrad(self, x0, g, mask, name, cin, cout, D, x1=None, c0=None):
@@ -224,11 +225,10 @@ class UNet3D:
         st = self._st()
         _lib.call("cm21_unet_in_fwd", _fp(delta), _fp(self.cond), _fp(p["in.w"]), _fp(p["in.b"]), _fp(self.beff),
                   _vp(a["e1a"]), self.B, D, ctypes.c_float(DELTA_SCALE), st)
-        self._conv(a["e1a"], self.wb["e1"], a["e1"], 16, 16, D, bias=p["e1.b"])
-        _lib.call("cm21_unet_pool_fwd", _vp(a["e1"]), _vp(a["p1"]), self.B, D, 16, st)
+        # each encoder level's last conv also writes the pooled box
+        self._conv(a["e1a"], self.wb["e1"], a["e1"], 16, 16, D, bias=p["e1.b"], pool=a["p1"])
         self._conv(a["p1"], self.wb["e2a"], a["e2a"], 16, 32, h, bias=p["e2a.b"])
-        self._conv(a["e2a"], self.wb["e2b"], a["e2"], 32, 32, h, bias=p["e2b.b"])
-        _lib.call("cm21_unet_pool_fwd", _vp(a["e2"]), _vp(a["p2"]), self.B, h, 32, st)
+        self._conv(a["e2a"], self.wb["e2b"], a["e2"], 32, 32, h, bias=p["e2b.b"], pool=a["p2"])
         self._conv(a["p2"], self.wb["b1"], a["b1"], 32, 64, q, bias=p["b1.b"])
         self._conv(a["b1"], self.wb["b2"], a["b2"], 64, 64, q, bias=p["b2.b"])
         _lib.call("cm21_unet_up_fwd", _vp(a["b2"]), _vp(self.wb["up2"]), _fp(p["up2.b"]), _vp(a["u2"]), self.B, q, 64,

@@ -295,6 +295,10 @@ typedef struct cm21_unet_conv {
                             (cout = 16, inference): prob receives
                             sigmoid(b + w . y) and y0 is not written; or NULL */
     float* prob;         /* fp32 (B,D,D,D) when head is set */
+    void* pool;          /* bf16 (B,D/2,D/2,D/2,cout) or NULL: also receives the
+                            2x2x2 mean of y0, equal to cm21_unet_pool_fwd of y0
+                            (fused into the sliding form for 16 -> 16 channels;
+                            needs o0 = cout and no head) */
 } cm21_unet_conv;
 /* y = conv(x) on the matrix cores; (cin, cout) in {(16,16), (16,32), (32,16),
  * (32,32), (32,64), (64,32), (64,64)}, D a multiple of 16.  The input

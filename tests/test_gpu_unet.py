@@ -417,3 +417,39 @@ def test_fused_head_matches_unfused(gpu, D):
         torch.cuda.synchronize()
         print(f"algo {algo}: max |diff| {float((prob - ref).abs().max()):.3g}", flush=True)
         assert torch.equal(prob, ref), f"algo {algo}"
+
+
+@pytest.mark.parametrize("D", [64, 128])
+def test_conv_pool_output_matches_pool_kernel(gpu, D):
+    """A conv given a pool buffer also writes the 2x2x2 mean of its output:
+    bit for bit what cm21_unet_pool_fwd makes of the stored output, for every
+    conv form (the sliding form of 16 -> 16 channels computes it in its
+    epilogue, the others pool afterwards), with the stored output unchanged;
+    with and without the ReLU."""
+    lib = gpu
+    B = 1 if D >= 128 else 2
+    st = lib.stream_handle()
+    for cin, cout in ((16, 16), (32, 32)):
+        x = _bf((B, D, D, D, cin), 1.0, 41)
+        w = _bf((cout, 27, cin), np.sqrt(2.0 / (27 * cin)), 42)
+        bias = _f((cout,), 0.1, 43)
+        for algo in (1, 2, 3):
+            for relu in ((True, False) if (cin, algo) == (16, 2) else (True,)):
+                print(f"pool fused: {cin}->{cout} D={D} algo={algo} relu={relu}", flush=True)
+                y_ref = torch.empty((B, D, D, D, cout), dtype=torch.bfloat16, device="cuda")
+                _conv_call(lib, B, D, cin, cout, x, w, y_ref, bias=bias, relu=relu, algo=algo)
+                p_ref = torch.empty((B, D // 2, D // 2, D // 2, cout), dtype=torch.bfloat16, device="cuda")
+                lib.call("cm21_unet_pool_fwd", _vp(y_ref), _vp(p_ref), B, D, cout, st)
+                y = torch.full_like(y_ref, float("nan"))
+                pooled = torch.full_like(p_ref, float("nan"))
+                c = lib.UnetConv(B=B, D=D, cin=cin, cout=cout, c0=cin, o0=cout, x0=_vp(x), w=_vp(w), bias=_fp(bias),
+                                 relu=1 if relu else 0, y0=_vp(y), algo=algo, pool=_vp(pooled))
+                lib.call("cm21_unet_conv3_fwd", ctypes.byref(c), st)
+                torch.cuda.synchronize()
+                assert torch.equal(y.view(torch.int16), y_ref.view(torch.int16)), f"y algo {algo}"
+                assert torch.equal(pooled.view(torch.int16), p_ref.view(torch.int16)), f"pool algo {algo}"
+    # the pool needs every output channel in y0 and no head
+    c = lib.UnetConv(B=B, D=D, cin=16, cout=16, c0=16, o0=16, x0=_vp(x), w=_vp(w), y0=_vp(y), pool=_vp(pooled),
+                     head=_fp(bias), prob=_fp(bias))
+    with pytest.raises(RuntimeError):
+        lib.call("cm21_unet_conv3_fwd", ctypes.byref(c), st)
