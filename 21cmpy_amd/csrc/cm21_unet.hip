@@ -1311,21 +1311,26 @@ __global__ __launch_bounds__(256) void k_in_fwd(const float* __restrict__ delta,
     const int y0 = (blockIdx.y % nyt) * TY, z0 = (blockIdx.y / nyt) * TZ;
     const int b = blockIdx.z;
     const size_t vb = (size_t)b * D * D * D;
-    // every halo load of the thread in flight before the LDS writes
-    constexpr int NH = HZ * HY * HX, NIT = (NH + 255) / 256;
+    const float* db = delta + vb;  // this box; 32-bit offsets inside it (D^3 < 2^31)
+    // halo rows (hz, hy) of HX voxels: thread t loads column t % HX of rows
+    // t / HX + RPI it (threads past RPI HX idle), every load in flight
+    // before the LDS writes
+    constexpr int NR = HZ * HY, RPI = 256 / HX, NIT = (NR + RPI - 1) / RPI;
+    const int hx = threadIdx.x % HX, hr0 = threadIdx.x / HX;
+    const unsigned gx = (unsigned)wrap(x0 + hx - 1, D);
     float hv[NIT];
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
-        const int i = threadIdx.x + it * 256;
-        if (i < NH) {
-            const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
-            hv[it] = delta[vb + ((size_t)wrap(z0 + hz - 1, D) * D + wrap(y0 + hy - 1, D)) * D + wrap(x0 + hx - 1, D)];
+        const int r = hr0 + it * RPI;
+        if (hr0 < RPI && r < NR) {
+            const int hz = r / HY, hy = r - hz * HY;
+            hv[it] = db[((unsigned)wrap(z0 + hz - 1, D) * D + (unsigned)wrap(y0 + hy - 1, D)) * D + gx];
         }
     }
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
-        const int i = threadIdx.x + it * 256;
-        if (i < NH) tile[i] = hv[it];
+        const int r = hr0 + it * RPI;
+        if (hr0 < RPI && r < NR) tile[r * HX + hx] = hv[it];
     }
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1336,7 +1341,9 @@ __global__ __launch_bounds__(256) void k_in_fwd(const float* __restrict__ delta,
     for (int s = 0; s < 7; ++s) {
         const int t = 4 * s + grp;
         wa[s] = t < 27 ? w[(row * 27 + t) * 3] * scale : 0.0f;
-        toff[s] = t < 27 ? ((t / 9) * HY + (t / 3) % 3) * HX + t % 3 : 0;
+        // tile offset of tap t: one of four compile-time constants
+        constexpr auto off = [](int tp) { return tp < 27 ? tap_off<HY, HX, 1>(tp) : 0; };
+        toff[s] = grp == 0 ? off(4 * s) : (grp == 1 ? off(4 * s + 1) : (grp == 2 ? off(4 * s + 2) : off(4 * s + 3)));
     }
     const floatx4 bias = *(const floatx4*)(beff + b * 16 + grp * 4);
     __syncthreads();
