@@ -210,7 +210,7 @@ def plain_rate(torch, delta, L, deltac, smin, zeta, steps, linear=True):
 def surrogate_bench(args, torch, plain_boxes_s=None):
     """U-Net surrogate at HII_DIM=128 (BOX_LEN=200 Mpc): inference boxes/s at
     batch 8, a training step's time, and accuracy against the plain method
-    on held-out seeds (3000..3015; trained on 1000..2999)."""
+    on held-out seeds (3000..3015; trained on 1000..2999 and 3200..7199)."""
     sur = importlib.import_module("21cmpy_amd.surrogate")
     D, L, B = 128, 200.0, 8
     trained = os.path.exists(sur.WEIGHTS)

@@ -330,7 +330,7 @@ def _heldout(n, D=128, first=3000):
 
 
 def test_trained_network_gains_over_untrained(gpu):
-    """The committed weights (trained on seeds 1000..2999) against an
+    """The committed weights (trained on seeds 1000..2999 and 3200..7199) against an
     untrained network of the same shape on held-out seeds: the trained one
     reproduces the plain method's neutral fraction and ionized cells, the
     untrained one does not."""
