@@ -367,26 +367,30 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
             if (c > 0) __syncthreads();
             load_tile(c * CH);
             __syncthreads();
+            // steps k = (family f, input line L); the LDS fragments run two
+            // steps ahead (across the family boundaries), since the one to
+            // three MFMAs of a step are shorter than the LDS latency
+            constexpr int NL = MW + 2, NS = 5 * NL;
+            bf16x8 ar[3];
+            ar[0] = fam_a(0, 0);
+            ar[1] = fam_a(0, 1);
 #pragma unroll
-            for (int f = 0; f < 5; ++f) {
-                if (f + 1 < 5) fam_w(c, f + 1, wq[(f + 1) & 1]);
-                bf16x8 acur = fam_a(f, 0), anext;
+            for (int k = 0; k < NS; ++k) {
+                const int f = k / NL, L = k % NL;
+                if (L == 0 && f + 1 < 5) fam_w(c, f + 1, wq[(f + 1) & 1]);
+                if (k + 2 < NS) ar[(k + 2) % 3] = fam_a((k + 2) / NL, (k + 2) % NL);
+                const bf16x8 acur = ar[k % 3];
 #pragma unroll
-                for (int L = 0; L < MW + 2; ++L) {
-                    if (L + 1 < MW + 2) anext = fam_a(f, L + 1);
+                for (int dy = 0; dy < 3; ++dy) {
+                    const int m = L - dy;
+                    if (m >= 0 && m < MW) {
 #pragma unroll
-                    for (int dy = 0; dy < 3; ++dy) {
-                        const int m = L - dy;
-                        if (m >= 0 && m < MW) {
-#pragma unroll
-                            for (int n = 0; n < NT; ++n)
-                                acc[m][n] =
-                                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(wq[f & 1][dy][n], acur, acc[m][n], 0, 0, 0);
-                        }
+                        for (int n = 0; n < NT; ++n)
+                            acc[m][n] =
+                                __builtin_amdgcn_mfma_f32_16x16x32_bf16(wq[f & 1][dy][n], acur, acc[m][n], 0, 0, 0);
                     }
-                    acur = anext;
-                    __builtin_amdgcn_sched_barrier(0);
                 }
+                __builtin_amdgcn_sched_barrier(0);
             }
             continue;
         }
