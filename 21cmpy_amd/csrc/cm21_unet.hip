@@ -244,13 +244,16 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
     // waves in turn; lane p < CPR of a wave loads 16-byte chunk p of the row
     // (voxel hx = p / (CH/8), channels ch .. ch+7), so only the row offset
     // changes between the rows.  All rows are in flight before the LDS
-    // writes; lanes past CPR load and write a copy of chunk CPR-1.
+    // writes; lanes past CPR load a copy of chunk CPR-1 and write nothing
+    // (their writes to one address would serialise in the LDS).
     auto load_tile = [&](int cb) {
         const bf16_t* lsrc[NP];
         const bf16_t* lmask[NP];
         int lmul[NP], llds[NP];
+        bool lw[NP];
 #pragma unroll
         for (int j = 0; j < NP; ++j) {
+            lw[j] = lane + 64 * j < CPR;
             const int p = min(lane + 64 * j, CPR - 1);
             const int hx = p / (CH / 8), ch = (p % (CH / 8)) * 8, gch = cb + ch;
             const unsigned gx = wrap(x0 + hx - 1, D);
@@ -280,7 +283,8 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
 #pragma unroll
             for (int it = 0; it < RW; ++it)
 #pragma unroll
-                for (int j = 0; j < NP; ++j) *(u32x4*)(tile + (wave + 4 * it) * HX * CP + llds[j]) = v[it][j];
+                for (int j = 0; j < NP; ++j)
+                    if (lw[j]) *(u32x4*)(tile + (wave + 4 * it) * HX * CP + llds[j]) = v[it][j];
         } else {
             for (int it = 0; it < RW; ++it) {
                 const unsigned rowv = row_off(it);
@@ -288,7 +292,7 @@ __global__ __launch_bounds__(256) void k_conv3_mfma(ConvArgs a) {
                 for (int j = 0; j < NP; ++j) {
                     const u32x4 v = *(const u32x4*)(lsrc[j] + (size_t)rowv * lmul[j]);
                     const u32x4 m = *(const u32x4*)(lmask[j] + (size_t)rowv * c0);
-                    *(u32x4*)(tile + (wave + 4 * it) * HX * CP + llds[j]) = mask8(v, m);
+                    if (lw[j]) *(u32x4*)(tile + (wave + 4 * it) * HX * CP + llds[j]) = mask8(v, m);
                 }
             }
         }
