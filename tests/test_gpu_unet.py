@@ -102,6 +102,41 @@ def test_conv3_fwd(gpu, cin, cout, D, split):
             _close_bf16(y1, ref[..., o0:], f"y1 algo {algo}")
 
 
+
+@pytest.mark.parametrize("cin,cout,D", [(16, 16, 32), (32, 16, 64), (32, 32, 64)])
+def test_conv3_fwd_chunk_across_inputs(gpu, cin, cout, D):
+    """Input split at c0 = 8: the 16-channel chunks of the chunked tile take
+    channels from both inputs (the general load path; at c0 a multiple of 16
+    every chunk lies in one input), also with the ReLU-mask input of the
+    input-gradient form on a single input."""
+    lib = gpu
+    B = 2
+    c0 = 8
+    x = _bf((B, D, D, D, cin), 0.7, 11)
+    w = _bf((cout, 27, cin), np.sqrt(2.0 / (27 * cin)), 12)
+    bias = _f((cout,), 0.1, 13)
+    x0, x1 = x[..., :c0].contiguous(), x[..., c0:].contiguous()
+    y = torch.empty((B, D, D, D, cout), dtype=torch.bfloat16, device="cuda")
+    ref = torch.stack([unet.conv3(_d(x[b:b + 1]), _d(w), _d(bias))[0] for b in range(B)])
+    for algo in ((1, 2, 3) if D >= 64 else (0, 3)):
+        print(f"conv fwd split 8 cin={cin} cout={cout} D={D} algo={algo}", flush=True)
+        y.fill_(float("nan"))
+        _conv_call(lib, B, D, cin, cout, x0, w, y, x1=x1, c0=c0, bias=bias, algo=algo)
+        torch.cuda.synchronize()
+        _close_bf16(y, ref, f"split 8 algo {algo}")
+    # mask on a single input: both load paths give the same bits
+    m = _bf((B, D, D, D, cin), 1.0, 14)
+    outs = []
+    for algo in ((1, 3) if D >= 64 else (0, 3)):
+        yy = torch.empty_like(y)
+        _conv_call(lib, B, D, cin, cout, x, w, yy, mask=m, relu=False, algo=algo)
+        outs.append(yy)
+    torch.cuda.synchronize()
+    keep = (_d(m) > 0).to(torch.float64)
+    refm = torch.stack([unet.conv3(_d(x[b:b + 1]) * keep[b:b + 1], _d(w), None, relu=False)[0] for b in range(B)])
+    for yy in outs:
+        _close_bf16(yy, refm, "masked input")
+
 # weight gradient: fp32 VALU form at D = 16, matrix-core form at D >= 32
 # (split: the input arrives as two tensors of cin/2 channels)
 @pytest.mark.parametrize("cin,cout,D,split", [(16, 16, 32, False), (32, 16, 16, False), (64, 32, 16, False),
