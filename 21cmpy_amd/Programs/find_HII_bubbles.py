@@ -16,6 +16,7 @@ none of the three given, they follow from Z, INIT_PARAMS.ION_Tvir_MIN and the
 Planck13 cosmology of INIT_PARAMS (cosmo_files.planck: sig_del, D(z), and
 S(R) of the Planck13 spectrum), as the reference computes them.
 """
+import functools
 import importlib
 import math
 import os
@@ -71,9 +72,14 @@ def barrier_denominators(radii, sigma_min2, sig0_fn):
     return out
 
 
+@functools.lru_cache(maxsize=16)
+def _default_spectrum(D, box_len):
+    return PowerLawSpectrum(D, box_len)
+
+
 def _default_sig0(D, box_len):
-    spec = PowerLawSpectrum(D, box_len)
-    return spec.sig0_discrete
+    # one spectrum per box shape, so S(R) of its radii is computed once
+    return _default_spectrum(int(D), float(box_len)).sig0_discrete
 
 
 def bubble_setup(D, box_len, scale, deltac, sigma_min2, sig0_fn, py2_int_scale=False):

@@ -59,6 +59,7 @@ class TableSpectrum:
             self.k = np.asarray(f["k"], np.float64)
             self.del2_table = np.asarray(f["Del2"], np.float64)
         self._lk = np.log(self.k)
+        self._s0 = {}  # S(R) by radius (the table does not change)
 
     def del2(self, k):
         k = np.asarray(k, dtype=np.float64)
@@ -66,7 +67,11 @@ class TableSpectrum:
 
     def sig0(self, R):
         """S(R) by the reference's rule (sigmas.py:176-191, 224-232)."""
-        return sig0_trapz(R, self.del2)
+        R = float(R)
+        s = self._s0.get(R)
+        if s is None:
+            s = self._s0[R] = sig0_trapz(R, self.del2)
+        return s
 
 
 def eh_nowiggle_transfer(k, cosmo=None, t_cmb=2.728):
@@ -98,13 +103,18 @@ class EHSpectrum:
         self.amp = 1.0
         s8 = sig0_trapz(8.0 / c["h"], self.del2)
         self.amp = c["sigma_8"] ** 2 / s8
+        self._s0 = {}  # S(R) by radius, once the amplitude is set
 
     def del2(self, k):
         k = np.asarray(k, dtype=np.float64)
         return self.amp * k ** (3.0 + self.n) * eh_nowiggle_transfer(k, self.cosmo) ** 2
 
     def sig0(self, R):
-        return sig0_trapz(R, self.del2)
+        R = float(R)
+        s = self._s0.get(R)
+        if s is None:
+            s = self._s0[R] = sig0_trapz(R, self.del2)
+        return s
 
 
 _TABLE_SPECTRUM = None

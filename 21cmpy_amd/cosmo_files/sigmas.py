@@ -74,6 +74,7 @@ class PowerLawSpectrum:
         self._k = k
         self._shape_k = self.shape(k)
         self.amp = self.sigma_cell2 * self.volume / float(np.sum(self._hist * self._shape_k))
+        self._s0 = {}  # S(R) by radius: the spectrum does not change after construction
 
     def shape(self, k):
         k = np.asarray(k, dtype=np.float64)
@@ -90,8 +91,12 @@ class PowerLawSpectrum:
         return k ** 3 * self.pk(k) / (2.0 * np.pi ** 2)
 
     def sig0_discrete(self, R):
-        w = tophat_window(self._k * float(R))
-        return float(np.sum(self._hist * self.amp * self._shape_k * w * w) / self.volume)
+        R = float(R)
+        s = self._s0.get(R)
+        if s is None:
+            w = tophat_window(self._k * R)
+            s = self._s0[R] = float(np.sum(self._hist * self.amp * self._shape_k * w * w) / self.volume)
+        return s
 
     def sig0_trapz(self, R, nint=1000):
         return sig0_trapz(R, self.del2, nint)

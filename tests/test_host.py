@@ -133,6 +133,24 @@ def test_default_sig0_matches_oracle():
         assert fn(R) == pytest.approx(sigma.sig0_discrete(R, 64, 100.0), rel=1e-12)
 
 
+def test_sig0_memo_returns_the_computed_values():
+    """S(R) is kept per radius on each spectrum object (the radius loop asks
+    for the same radii on every call); the kept values are the computed ones,
+    and the default spectrum of a box shape is built once."""
+    fb = pkg("Programs.find_HII_bubbles")
+    sig = pkg("cosmo_files.sigmas")
+    pl = pkg("cosmo_files.planck")
+    r, _ = fb.radius_schedule(300.0, 256)
+    spec = sig.PowerLawSpectrum(256, 300.0)
+    first = [spec.sig0_discrete(float(R)) for R in r]
+    again = [spec.sig0_discrete(np.float32(R)) for R in r]
+    fresh = [sig.PowerLawSpectrum(256, 300.0).sig0_discrete(float(R)) for R in r[:5]]
+    assert first == again and first[:5] == fresh
+    assert fb._default_sig0(256, 300).__self__ is fb._default_sig0(256.0, 300.0).__self__
+    tab = pl.planck_spectrum()
+    assert tab.sig0(5.0) == tab.sig0(5.0) == pl.sig0_trapz(5.0, tab.del2)
+
+
 def test_conv_bubbles_fails_loudly_without_gpu():
     import torch
     if torch.cuda.is_available():
