@@ -101,3 +101,51 @@ def test_commutes_with_axis_permutation(seed, smin2, zeta, perm):
     g = np.transpose(bubbles.find_bubbles_f32(delta, L, RADII, den, zeta, 1.686), perm)
     ok32 = (mw > 1e-5) & (m_p > 1e-5)
     assert np.array_equal((f == 1)[ok32], (g == 1)[ok32])
+
+
+# ---------------------------------------------------------------- P21(k)
+# deltat.power_spectrum (delta_T.py:80-98 with the conventions kept from
+# delta_T.cu:5-29): the binned power of delta_T/<delta_T> - 1 does not change
+# under a global rescaling of the field (the dimensional one scales as its
+# square), under periodic shifts and under permutations of the axes, which
+# moves the halved r2c axis (a wrong weight of its 0 and Nyquist planes
+# fails).
+
+from oracle import deltat  # noqa: E402
+
+PD, PL = 16, 50.0
+EDGES = deltat.log_edges(PL, PD, 12)
+
+
+def _field(seed):
+    rng = np.random.default_rng(seed)
+    return 1.0 + 0.3 * rng.standard_normal((PD, PD, PD))
+
+
+def _ps(f, dimensional=False):
+    return deltat.power_spectrum(f, PL, EDGES, dimensional=dimensional)
+
+
+@COMMON
+@given(seed=SEEDS, a=st.floats(0.01, 100.0))
+def test_power_spectrum_scaling(seed, a):
+    f = _field(seed)
+    k0, p0, c0, _ = _ps(f)
+    k1, p1, c1, _ = _ps(a * f)
+    assert np.array_equal(c0, c1) and np.array_equal(k0, k1)
+    assert np.allclose(p1, p0, rtol=1e-10, atol=0)
+    pd0, pd1 = _ps(f, True)[1], _ps(a * f, True)[1]
+    assert np.allclose(pd1, a * a * pd0, rtol=1e-10, atol=0)
+
+
+@COMMON
+@given(seed=SEEDS, perm=st.permutations([0, 1, 2]),
+       shift=st.tuples(st.integers(0, PD - 1), st.integers(0, PD - 1), st.integers(0, PD - 1)))
+def test_power_spectrum_shift_and_axis_invariance(seed, perm, shift):
+    f = _field(seed)
+    k0, p0, c0, _ = _ps(f)
+    for g in (np.roll(f, shift, axis=(0, 1, 2)), np.ascontiguousarray(np.transpose(f, perm))):
+        k1, p1, c1, _ = _ps(g)
+        assert np.array_equal(c0, c1)
+        assert np.allclose(k1, k0, rtol=1e-12, atol=0)
+        assert np.allclose(p1, p0, rtol=1e-9, atol=0)
