@@ -149,3 +149,29 @@ def test_power_spectrum_shift_and_axis_invariance(seed, perm, shift):
         assert np.array_equal(c0, c1)
         assert np.allclose(k1, k0, rtol=1e-12, atol=0)
         assert np.allclose(p1, p0, rtol=1e-9, atol=0)
+
+
+# ---------------------------------------------------------------- mass move
+# perturb.move_mass_ngp / _cic (perturb_field.py:65-108, perturb_field.cu:
+# 90-133): permuting the axes of the density and of the displacement fields
+# (component a of the new field is the permuted component perm[a]) permutes
+# the moved box; mixing up displacement components or the axis a component's
+# lattice term is laid along fails.
+
+from oracle import perturb  # noqa: E402
+
+
+@COMMON
+@given(seed=SEEDS, perm=st.permutations([0, 1, 2]), sw=st.sampled_from([4, 8]), scheme=st.sampled_from(["ngp", "cic"]))
+def test_mass_move_commutes_with_axis_permutation(seed, perm, sw, scheme):
+    rng = np.random.default_rng(seed)
+    DIM = 8
+    delta = rng.standard_normal((DIM, DIM, DIM)).astype(np.float32)
+    v = [(rng.standard_normal((sw, sw, sw)) * 3.0).astype(np.float32) for _ in range(3)]
+    fn = perturb.move_mass_ngp if scheme == "ngp" else perturb.move_mass_cic
+    args = (0.5, 0.1, 10.0)  # growth, initial growth, box length: displacements of about a cell
+    base = fn(delta, *v, *args)
+    pd = np.ascontiguousarray(np.transpose(delta, perm))
+    pv = [np.ascontiguousarray(np.transpose(v[perm[a]], perm)) for a in range(3)]
+    got = fn(pd, *pv, *args)
+    assert np.allclose(got, np.transpose(base, perm), rtol=1e-6, atol=1e-6)
