@@ -24,7 +24,10 @@ Pinning status of each module (what checks it besides the GPU):
   mode sum pinned by brute force on small grids and by its R->0 limit.
 * ``bubbles``  - filter windows pinned by closed forms and a brute-force DFT;
   the loop by constant-box closed forms, OR-monotonicity and a pure-Python
-  loop restatement on tiny boxes.
+  loop restatement on tiny boxes; the nonlinear-input branch (growth scaling
+  and box-mean normalisation) by the constant-box value zeta*fc_mean, the
+  per-radius box mean fc_mean, growth-only-through-the-input, one scalar per
+  radius and zeta-monotonicity, each shown to catch a planted mistake.
 * ``grf``      - Hermitian symmetry (irfftn round trip), variances, Philox.
 * ``perturb``  - NGP indices by a pure-Python loop restatement, mass
   conservation, zero-displacement block sums, integer-shift rolls.

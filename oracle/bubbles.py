@@ -125,6 +125,33 @@ def fcoll_f32(smoothed, deltac, denom):
     return erfc_f32(_f32(_f32(np.float32(deltac) - _f32(smoothed)) / np.float32(denom)))
 
 
+def nonlinear_input_f32(d, growth_inv):
+    """Linear extrapolation of a nonlinear density box to z = 0,
+    find_HII_bubbles.py:58,84-85: deltax *= float32(deltac/1.686)."""
+    return _f32(_f32(d) * np.float32(growth_inv))
+
+
+def nonlinear_input_f64(d, growth_inv):
+    """FP64 counterpart of nonlinear_input_f32."""
+    return np.asarray(d, dtype=np.float64) * float(growth_inv)
+
+
+def box_mean_scale_f32(fc, fc_mean):
+    """Normalisation factor of find_HII_bubbles.py:158-161,171-174:
+    float32(fc_mean / <f_coll>) with the box mean <f_coll> = sum/N accumulated
+    in FP64 (the reference sums in float32 on the GPU; the build keeps FP64)."""
+    mean = np.float64(np.sum(np.asarray(fc, dtype=np.float64))) / np.asarray(fc).size
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        # a box mean of 0 gives an infinite scale (IEEE), as in the reference
+        return np.float32(np.float64(np.float32(fc_mean)) / mean)
+
+
+def box_mean_scale_f64(fc, fc_mean):
+    """FP64 counterpart of box_mean_scale_f32."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.float64(fc_mean) / np.float64(np.mean(fc))
+
+
 def paint_spheres(ionized, centers, R, periodic=False):
     """Sphere paint of find_bubbles.cu:83-101 for a boolean ``centers`` mask.
 
@@ -167,9 +194,8 @@ def find_bubbles_f32(delta, box_len, radii, denoms, zeta, deltac, LE=True,
     """
     d = _f32(delta)
     D = d.shape[0]
-    N = D ** 3
     if not LE:
-        d = _f32(d * np.float32(growth_inv))
+        d = nonlinear_input_f32(d, growth_inv)
     spec = np.fft.rfftn(d.astype(np.float64)).astype(np.complex64)
     kmag = kmag_half_f32(D, delta_k_f32(box_len), fastest_first=True)
     ionized = np.zeros((D, D, D), dtype=np.float32)
@@ -184,9 +210,9 @@ def find_bubbles_f32(delta, box_len, radii, denoms, zeta, deltac, LE=True,
         fc = fcoll_f32(sm, deltac, denom)
         scale = None
         if not LE:
-            mean = float(np.sum(fc.astype(np.float64))) / N
-            scale = np.float32(float(np.float32(fc_mean)) / mean)
-            fc = _f32(fc * scale)
+            scale = box_mean_scale_f32(fc, fc_mean)
+            with np.errstate(invalid="ignore"):
+                fc = _f32(fc * scale)
         hit = fc.astype(np.float64) >= inv_zeta
         if return_trace:
             trace.append(dict(R=R, smoothed=sm, fcoll=fc, scale=scale))
@@ -220,7 +246,7 @@ def find_bubbles_f64(delta, box_len, radii, denoms, zeta, deltac, LE=True,
     d = np.asarray(delta, dtype=np.float64)
     D = d.shape[0]
     if not LE:
-        d = d * float(growth_inv)
+        d = nonlinear_input_f64(d, growth_inv)
     spec = rfftn(d)
     dk = 2.0 * np.pi / float(box_len)
     n0 = mode_index(D).astype(np.float64)
@@ -234,7 +260,8 @@ def find_bubbles_f64(delta, box_len, radii, denoms, zeta, deltac, LE=True,
         sm = irfftn(spec * w, (D, D, D))
         fc = _erfc((float(deltac) - sm) / float(denom))
         if not LE:
-            fc = fc * (float(fc_mean) / fc.mean())
+            with np.errstate(invalid="ignore"):
+                fc = fc * box_mean_scale_f64(fc, fc_mean)
         margin = np.minimum(margin, np.abs(zeta * fc - 1.0))
         hit = fc >= 1.0 / zeta
         if r == nR - 1:
