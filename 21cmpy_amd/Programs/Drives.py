@@ -9,10 +9,13 @@ and BOX_LEN, so they form one group with their own barriers 1.686/D(z)),
 followed by the brightness temperature and P21(k) of each redshift.
 
 Without ``sigma_min2`` the run follows the Planck13 cosmology of INIT_PARAMS
-as the reference does with cosmolopy: D(z) and, per redshift, the barrier and
-S(R) from cosmo_files.planck (ION_Tvir_MIN).  With an explicit ``sigma_min2``
-the growth factors come from ``growth`` (a callable, default D(z) = 0.12 * 9 /
-(1 + z), anchored at D(8) = 0.12) and every redshift uses that sigma_min^2.
+as the reference does with cosmolopy: the initial conditions drawn from the
+Planck13 P(k), D(z) and, per redshift, the barrier and S(R) of the same
+spectrum from cosmo_files.planck (ION_Tvir_MIN).  With an explicit ``sigma_min2``
+the initial conditions follow the benchmark power law, the growth factors
+come from ``growth`` (a callable, default D(z) = 0.12 * 9 / (1 + z), anchored
+at D(8) = 0.12) and every redshift uses that sigma_min^2.  The mass move is
+cloud-in-cell by default (``scheme``; "ngp" is the reference's truncation).
 """
 import numpy as np
 
@@ -35,7 +38,10 @@ def drive_zscroll_noTs(ZHIGH=16.0, ZLOW=10.0, DZ=-2.0, DIM=None, HII_DIM=None, B
     zeta = float(P.ZETA if zeta is None else zeta)
     if not quiet:
         print("Now calling initialize")
-    ics = initialize.init(DIM=DIM, HII_DIM=HII_DIM, BOX_LEN=L, seed=seed, save=save, out_dir=out_dir)
+    # one cosmology throughout: the Planck13 spectrum with the Planck13 barrier,
+    # the benchmark power law with an explicit sigma_min^2
+    ics = initialize.init(DIM=DIM, HII_DIM=HII_DIM, BOX_LEN=L, seed=seed, pspec="planck" if planck_mode else "powerlaw",
+                          save=save, out_dir=out_dir)
     zs = np.arange(ZHIGH, ZLOW, DZ)
     d_init = grow(P.INITIAL_REDSHIFT)
     dens = []
@@ -43,7 +49,7 @@ def drive_zscroll_noTs(ZHIGH=16.0, ZLOW=10.0, DZ=-2.0, DIM=None, HII_DIM=None, B
         if not quiet:
             print(z)
         dens.append(perturb_field.evolve_zeldovich(z, ics["deltax"], ics["v"], grow(z), d_init, L, scheme,
-                                                   save=save, out_dir=out_dir))
+                                                   save=save, out_dir=out_dir, velocities=False))
     out = {"z": zs, "density": dens}
     if not bubbles:
         return out

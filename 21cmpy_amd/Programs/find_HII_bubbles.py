@@ -34,19 +34,23 @@ DELTA_C_LINEAR = 1.686
 
 
 def radius_schedule(box_len, hii_dim, py2_int_scale=False, r_max=None, r_min=None, delta_r_factor=None,
-                    l_factor=None):
+                    l_factor=None, cell=None):
     """float32 radii R_max, R_max/1.1, ... and the cell size of the stop rule.
 
-    find_HII_bubbles.py:53-54 (scale; integer division under Python 2 when
-    py2_int_scale), :92, :123-125 (final_step when R/1.1 <= L_FACTOR*scale or
-    <= R_BUBBLE_MIN, tested in float64), :125 (R cast to float32), :189
-    (R/1.1 in float64).  The last radius returned is the final step.
+    find_HII_bubbles.py:52-54 (scale: the caller's ``cell`` if given, else
+    BoxSize/HIIdim, an integer division under Python 2 when py2_int_scale),
+    :92, :123-125 (final_step when R/1.1 <= L_FACTOR*scale or <=
+    R_BUBBLE_MIN, tested in float64), :125 (R cast to float32), :189 (R/1.1
+    in float64).  The last radius returned is the final step.
     """
     r_max = P.R_BUBBLE_MAX if r_max is None else r_max
     r_min = P.R_BUBBLE_MIN if r_min is None else r_min
     fac = P.DELTA_R_FACTOR if delta_r_factor is None else delta_r_factor
     lf = float(np.float32(P.L_FACTOR if l_factor is None else l_factor))
-    scale = (int(box_len) // int(hii_dim)) if py2_int_scale else float(box_len) / float(hii_dim)
+    if cell is not None:
+        scale = float(cell)
+    else:
+        scale = (int(box_len) // int(hii_dim)) if py2_int_scale else float(box_len) / float(hii_dim)
     R = float(r_max)
     radii = []
     while True:
@@ -84,9 +88,7 @@ def _default_sig0(D, box_len):
 
 def bubble_setup(D, box_len, scale, deltac, sigma_min2, sig0_fn, py2_int_scale=False):
     """Host-side scalars of conv_bubbles: radii, denominators, fc_mean."""
-    radii, cell = radius_schedule(box_len, D, py2_int_scale)
-    if scale is not None:
-        cell = scale
+    radii, cell = radius_schedule(box_len, D, py2_int_scale, cell=scale)
     lf = float(np.float32(P.L_FACTOR))
     if P.R_BUBBLE_MIN < lf * cell:  # find_HII_bubbles.py:115-120
         if 2.0 * (float(sigma_min2) - sig0_fn(lf * cell)) < 0:
@@ -156,35 +158,51 @@ def conv_bubbles(deltax, param_dict, Z=None, scale=None, fil=1, update=0, LE=Fal
         graph=1 if graph else 0)
     is_tensor = isinstance(deltax, torch.Tensor)
     dev = torch.device("cuda")
-    d = deltax if is_tensor else torch.from_numpy(np.ascontiguousarray(deltax, dtype=np.float32))
-    d = d.to(dev, dtype=torch.float32).contiguous()
+    caller = torch.cuda.current_stream()
+    s = caller if stream is None else stream
     plan = _lib.plan(D, 1)
     nbytes = _lib.query("cm21_find_bubbles_workspace_bytes", D, len(radii), int(update))
-    if graph:
-        # a replayed graph reuses the captured addresses: keep the buffers
-        ion, work = _graph_buffers(d, nbytes)
-    else:
-        ion = torch.empty_like(d)
-        work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     r = np.ascontiguousarray(radii, dtype=np.float32)
     dn = np.ascontiguousarray(denoms, dtype=np.float32)
-    _lib.call("cm21_find_bubbles", plan.handle, _lib.ptr(d), _lib.ptr(ion),
-              r.ctypes.data_as(_lib.c_float_p), dn.ctypes.data_as(_lib.c_float_p), len(radii),
-              prm, _lib.ptr(work), nbytes, _lib.stream_handle(stream))
-    if is_tensor:
-        return ion.clone() if graph else ion
-    return ion.cpu().numpy()
+    # Everything runs on `s`, ordered after the caller's pending work and
+    # before the caller's later work: the input upload, the loop, the read-back.
+    if s != caller:
+        s.wait_stream(caller)
+    with torch.cuda.stream(s):
+        d = deltax if is_tensor else torch.from_numpy(np.ascontiguousarray(deltax, dtype=np.float32))
+        d = d.to(dev, dtype=torch.float32).contiguous()
+        if graph:
+            # a replayed graph reuses the captured addresses: keep the buffers
+            ion, work = _graph_buffers(d, nbytes, s)
+        else:
+            ion = torch.empty_like(d)
+            work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        _lib.call("cm21_find_bubbles", plan.handle, _lib.ptr(d), _lib.ptr(ion),
+                  r.ctypes.data_as(_lib.c_float_p), dn.ctypes.data_as(_lib.c_float_p), len(radii),
+                  prm, _lib.ptr(work), nbytes, _lib.stream_handle(s))
+        out = ion.clone() if (graph and is_tensor) else ion
+        host = None if is_tensor else out.cpu().numpy()
+    if s != caller:
+        caller.wait_stream(s)
+        for t in (d, out, work):
+            t.record_stream(caller)
+    return out if is_tensor else host
 
 
+# Graph replay needs the captured buffers to stay alive.  One entry is kept per
+# (box side, stream), the granularity at which the library keeps graphs: a new
+# input address replaces the entry (and its buffers) of its shape and stream.
 _graph_cache = {}
 
 
-def _graph_buffers(d, nbytes):
-    """Output and workspace kept alive per (input address, shape) for graph replay."""
-    key = (d.data_ptr(), tuple(d.shape), nbytes)
-    if key not in _graph_cache:
-        _graph_cache[key] = (torch.empty_like(d), torch.empty(nbytes, dtype=torch.uint8, device=d.device))
-    return _graph_cache[key]
+def _graph_buffers(d, nbytes, stream):
+    """Output and workspace kept alive for graph replay of the input `d`."""
+    slot = (tuple(d.shape), nbytes, stream.cuda_stream)
+    cur = _graph_cache.get(slot)
+    if cur is None or cur[0] != d.data_ptr():
+        _graph_cache[slot] = (d.data_ptr(), torch.empty_like(d),
+                              torch.empty(nbytes, dtype=torch.uint8, device=d.device))
+    return _graph_cache[slot][1], _graph_cache[slot][2]
 
 
 def xH_filename(Z, nf, fil, dim, box_size):

@@ -193,8 +193,25 @@ class BatchedBubbles:
         return self._groups[gi]["delta"][j]
 
     def run(self):
+        """Compute every ionized box.  With a stream of its own the run waits
+        for the caller's current stream (the set_delta copies) and the
+        caller's stream waits for the run (its reads of the results)."""
+        caller = torch.cuda.current_stream()
+        s = caller if self.stream is None else self.stream
+        if s != caller:
+            s.wait_stream(caller)
         _lib.call("cm21_find_bubbles_batched", self._garr, len(self._groups), self.prm, _lib.ptr(self.work),
-                  self.work_bytes, _lib.stream_handle(self.stream))
+                  self.work_bytes, _lib.stream_handle(s))
+        if s != caller:
+            caller.wait_stream(s)
+            for t in self._buffers():
+                t.record_stream(s)
+
+    def _buffers(self):
+        for g in self._groups:
+            yield g["delta"]
+            yield g["ion"]
+        yield self.work
 
     def ionized(self, i):
         gi, j = self._slot[i]
@@ -218,7 +235,10 @@ class BatchedBubbles:
         ncand = sum(_lib.query("cm21_batch_tile_count", g["D"], 0) + _lib.query("cm21_batch_tile_count", g["D"], 1)
                     for g in self._groups)
         ms = (_lib.ctypes.c_double * max(ncand, 1))()
-        stream = self.stream if self.stream is not None else torch.cuda.current_stream()
+        caller = torch.cuda.current_stream()
+        stream = self.stream if self.stream is not None else caller
+        if stream != caller:
+            stream.wait_stream(caller)
         _lib.call("cm21_batch_autotune", self._garr, len(self._groups), self.prm, _lib.ptr(self.work),
                   self.work_bytes, int(reps), ms, _lib.stream_handle(stream))
         res, pos = {}, 0

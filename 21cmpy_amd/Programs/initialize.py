@@ -8,9 +8,14 @@ is drawn directly as a Hermitian half spectrum on the GPU (cm21_grf_init);
 everything stays in device memory and the .npy files are written only when
 ``save`` is set, under the reference's names (initialize.py:82,93,109).
 
-The power spectrum is the configurations' P(k) = A k^n exp(-(k/k_nyq)^2) with
-A set by the per-cell variance (cosmo_files.sigmas.PowerLawSpectrum); the
-reference's Eisenstein-Hu table (initialize.py:27-31) needs cosmolopy.
+The power spectrum is, by default, the reference's: the Planck13
+(INIT_PARAMS.COSMO) Eisenstein & Hu P(k) at z = 0 tabulated at 512 points
+log-spaced from DELTA_K/10 to DELTA_K sqrt(3) DIM (initialize.py:27-31) and
+interpolated linearly per mode on the GPU (cm21_grf_init_table), with the
+spectrum computed by cosmo_files.planck instead of cosmolopy.
+``pspec="powerlaw"`` gives the benchmark configurations' P(k) = A k^n
+exp(-(k/k_nyq)^2) with A set by the per-cell variance
+(cosmo_files.sigmas.PowerLawSpectrum).
 """
 import os
 
@@ -19,7 +24,10 @@ import torch
 
 from .. import gpu, _lib
 from ..Parameter_files import INIT_PARAMS as P
+from ..cosmo_files import planck
 from ..cosmo_files.sigmas import PowerLawSpectrum
+
+BLOCK_VOLUME = 512  # points of the P(k) table (initialize.py:28)
 
 
 def smoothing_radius(box_len, hii_dim):
@@ -36,23 +44,44 @@ def box_names(DIM, HII_DIM, BOX_LEN):
     }
 
 
-def init(DIM=None, HII_DIM=None, BOX_LEN=None, seed=0, box_id=0, sigma_cell2=0.36, n_index=-2.5,
+def pk_table(DIM, BOX_LEN, spectrum=None):
+    """float32 (K, P) table of init_pspec (initialize.py:27-31) for a spectrum
+    object (default: the Planck13 spectrum of INIT_PARAMS.COSMO)."""
+    sp = planck.planck_spectrum() if spectrum is None else spectrum
+    return sp.init_table(int(DIM), float(BOX_LEN), BLOCK_VOLUME)
+
+
+def init(DIM=None, HII_DIM=None, BOX_LEN=None, seed=0, box_id=0, pspec="planck", sigma_cell2=0.36, n_index=-2.5,
          with_velocity=True, save=True, out_dir=None):
-    """Returns {'spec', 'deltax', 'smoothed', 'v': (vx, vy, vz)} as CUDA tensors."""
+    """Returns {'spec', 'deltax', 'smoothed', 'v': (vx, vy, vz)} as CUDA tensors
+    (and 'pk_table' for a tabulated spectrum).
+
+    pspec: "planck" (the reference's spectrum, default), "powerlaw" (the
+    benchmark recipe with sigma_cell2 and n_index), or an object with
+    ``init_table(DIM, BOX_LEN, n)`` returning float32 (K, P)."""
     _lib.require_gpu()
     DIM = int(P.DIM if DIM is None else DIM)
     HII_DIM = int(P.HII_DIM if HII_DIM is None else HII_DIM)
     L = float(P.BOX_LEN if BOX_LEN is None else BOX_LEN)
     if DIM % HII_DIM:
         raise ValueError(f"HII_DIM={HII_DIM} must divide DIM={DIM}")
-    spectrum = PowerLawSpectrum(DIM, L, sigma_cell2, n_index)
-    spec = gpu.grf_init(DIM, L, spectrum.amp, seed, box_id, n_index, spectrum.k_cut)
+    table = None
+    if isinstance(pspec, str) and pspec == "powerlaw":
+        spectrum = PowerLawSpectrum(DIM, L, sigma_cell2, n_index)
+        spec = gpu.grf_init(DIM, L, spectrum.amp, seed, box_id, n_index, spectrum.k_cut)
+    else:
+        if isinstance(pspec, str) and pspec != "planck":
+            raise ValueError(f"pspec must be 'planck', 'powerlaw' or a spectrum object, not {pspec!r}")
+        table = pk_table(DIM, L, None if isinstance(pspec, str) else pspec)
+        spec = gpu.grf_init_table(DIM, L, table[0], table[1], seed, box_id)
     deltax = gpu.c2r(spec)                                   # initialize.py:79-82
     R = smoothing_radius(L, HII_DIM)
     work = gpu.hii_filter(spec, L, R, filter_type=0, ref_fallthrough=True, korder=1)
     large = gpu.c2r(work, preserve_input=False)              # initialize.py:86-93
     smoothed = gpu.subsample(large, HII_DIM)
     out = {"spec": spec, "deltax": deltax, "smoothed": smoothed}
+    if table is not None:
+        out["pk_table"] = table
     if with_velocity:
         vs = []
         for comp in range(3):                                # initialize.py:102-109

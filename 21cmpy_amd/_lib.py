@@ -120,6 +120,7 @@ SIGNATURES = {
     "cm21_find_bubbles_workspace_bytes": (SZ, [I, I, I]),
     "cm21_find_bubbles": (I, [VP, VP, VP, c_float_p, c_float_p, I, ctypes.POINTER(BubbleParams), VP, SZ, VP]),
     "cm21_grf_init": (I, [VP, I, D, D, D, D, U32, U32, VP]),
+    "cm21_grf_init_table": (I, [VP, I, D, VP, VP, I, U32, U32, VP]),
     "cm21_subsample": (I, [VP, VP, I, I, F, VP]),
     "cm21_velocity": (I, [VP, VP, I, F, I, I, F, VP]),
     "cm21_move_mass": (I, [VP, VP, VP, VP, VP, I, I, F, F, I, VP]),

@@ -5,6 +5,7 @@ Usage: python 21cmpy_amd/build.py [-v]
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -41,22 +42,30 @@ def hipcc():
     return "hipcc"
 
 
-def build(verbose=False):
-    """Compile every source to an object and link the shared library."""
+def build(verbose=False, jobs=None):
+    """Compile every source to an object (in parallel) and link the shared
+    library.  A missing source is an error."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
     inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
+    missing = [s for s in SOURCES if not os.path.exists(os.path.join(CSRC, s))]
+    if missing:
+        raise FileNotFoundError(f"missing HIP sources: {missing}")
+    cmds, objs = [], []
     for src in SOURCES:
-        path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = [hipcc(), *HIPCC_FLAGS, *inc, "-c", path, "-o", obj]
+        cmds.append([hipcc(), *HIPCC_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+
+    def compile_one(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    jobs = jobs or min(len(cmds), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(compile_one, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp,
            "-L/opt/rocm/lib", "-lrocfft", "-Wl,--no-undefined"]

@@ -6,16 +6,25 @@ The reference obtains the excursion-set barrier from cosmolopy
     sigmamin, deltac = pb.sig_del(ION_Tvir_MIN, Z, **COSMO)
     fc_mean_ps = pb.collapse_fraction(sigmamin, deltac)
 
-and S(R) from its cached Planck13 Delta^2(k) table (cosmo_files/sigmas.py:
-44-52, 224-232).  This module restates those quantities from their published
-definitions, for COSMO = PLANCK13 (Parameter_files/INIT_PARAMS.py):
+its S(R) from a cached Planck13 Delta^2(k) table (cosmo_files/sigmas.py:
+44-52, 224-232) and its initial-condition spectrum from
+pb.power_spectrum(K, 0.0, **COSMO) (initialize.py:27-31).  This module
+restates those quantities from their published definitions, for COSMO =
+PLANCK13 (Parameter_files/INIT_PARAMS.py):
 
-* Delta^2(k) at z = 0: the reference's own cached table (Del2k_planck13.npz,
-  written by make_planck_table.py), interpolated in log k as the reference's
-  interp1d does on its grid; or, analytically, the Eisenstein & Hu (1998,
-  ApJ 496, 605, eqs. 29-31) zero-baryon-oscillation fit normalised to
-  sigma_8 (``EHSpectrum``; within 3 % of the table, which carries
-  baryon oscillations).
+* P(k) at z = 0: Eisenstein & Hu (1999, ApJ 511, 5) transfer function for
+  cold dark matter, baryons and massive neutrinos (``eh99_transfer``), the fit
+  cosmolopy wraps, evaluated as the authors' public fitting code does it
+  (T_cmb = 2.728 K, an exponent 1.11 in the C(q) term, the baryon and
+  neutrino densities floored at Omega = 1e-5 and at least one neutrino
+  species, so that with N_nu = 0 a vanishing neutrino fraction 1e-5/Omega_m
+  enters), times k^n and normalised to sigma_8 by an adaptive quadrature of
+  the top-hat variance at 8/h Mpc (``EH99Spectrum``).  Against the
+  reference's cached Del2k.npz the shape agrees to 6e-7 and the amplitude to
+  8e-5.  S(R) follows the reference's rule on top of it: Delta^2 tabulated on
+  k = exp(arange(-21, 21, 0.1)), interpolated linearly in k, trapezoid rule
+  over ln k (sigmas.sig0_trapz).  The Eisenstein & Hu (1998, ApJ 496, 605,
+  eqs. 29-31) zero-baryon-oscillation fit is kept as ``EHSpectrum``.
 * D(z): Carroll, Press & Turner (1992, ARA&A 30, 499) fit for a flat
   Lambda universe, normalised to D(0) = 1 (the approximation cosmolopy's
   fgrowth documents), and the Liddle et al. fit the reference's own C code
@@ -31,16 +40,18 @@ definitions, for COSMO = PLANCK13 (Parameter_files/INIT_PARAMS.py):
 * collapse_fraction(sigma_min, delta_c) = erfc(delta_c / (sqrt(2) sigma_min)).
 """
 import math
-import os
 
 import numpy as np
+from scipy import integrate
 
 from ..Parameter_files import INIT_PARAMS as P
-from .sigmas import sig0_trapz
+from .sigmas import sig0_trapz, tophat_window
 
 DELTA_C_LINEAR = 1.686
 RHO_CRIT_H2 = 2.775e11  # M_sun Mpc^-3 h^-2 (comoving critical density today)
-TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "Del2k_planck13.npz")
+T_CMB = 2.728  # K, as in the Eisenstein & Hu fitting code
+# k grid of the reference's cached Delta^2(k) (cosmo_files/sigmas.py:50)
+REFERENCE_LOGK = np.arange(-21.0, 21.0, 0.1)
 
 
 def cosmo_params(cosmo=None):
@@ -50,31 +61,65 @@ def cosmo_params(cosmo=None):
 
 # ------------------------------------------------------------------ spectrum
 
-class TableSpectrum:
-    """Delta^2(k) at z = 0 from the cached Planck13 table (k in Mpc^-1),
-    interpolated linearly in ln k; zero outside the table."""
+def eh99_transfer(k, cosmo=None, t_cmb=T_CMB):
+    """Eisenstein & Hu (1999) transfer function of the cold dark matter plus
+    baryon component at z = 0 (k in Mpc^-1).
 
-    def __init__(self, path=TABLE):
-        with np.load(path, allow_pickle=False) as f:
-            self.k = np.asarray(f["k"], np.float64)
-            self.del2_table = np.asarray(f["Del2"], np.float64)
-        self._lk = np.log(self.k)
-        self._s0 = {}  # S(R) by radius (the table does not change)
+    Equation numbers refer to the paper: z_eq (1), z_d (2), y_d (3), the sound
+    horizon fit s (4), q (5), p_c and p_cb (11), the growth factor with
+    free-streaming (10, 12-14), the small-scale suppression alpha_nu (15), the
+    effective shape Gamma_eff (16), the suppressed transfer function (17-21)
+    and the neutrino master correction (22-24).
+    """
+    c = cosmo_params(cosmo)
+    h = float(c["h"])
+    om = float(c["omega_M_0"])
+    ol = float(c.get("omega_lambda_0", 1.0 - om))
+    ob = max(float(c["omega_b_0"]), 1.0e-5)
+    onu = max(float(c.get("omega_n_0", 0.0)), 1.0e-5)
+    n_nu = max(float(c.get("N_nu", 0)), 1.0)
+    theta2 = (t_cmb / 2.7) ** 2
+    w_m, w_b = om * h * h, ob * h * h
+    f_b, f_nu = ob / om, onu / om
+    f_c = 1.0 - f_b - f_nu
+    f_cb, f_nub = f_c + f_b, f_nu + f_b
 
-    def del2(self, k):
-        k = np.asarray(k, dtype=np.float64)
-        return np.interp(np.log(k), self._lk, self.del2_table, left=0.0, right=0.0)
+    one_z_eq = 2.5e4 * w_m / (theta2 * theta2)
+    d1 = 0.313 * w_m ** -0.419 * (1.0 + 0.607 * w_m ** 0.674)
+    d2 = 0.238 * w_m ** 0.223
+    z_d = 1291.0 * w_m ** 0.251 / (1.0 + 0.659 * w_m ** 0.828) * (1.0 + d1 * w_b ** d2)
+    y_d = one_z_eq / (1.0 + z_d)
+    s = 44.5 * math.log(9.83 / w_m) / math.sqrt(1.0 + 10.0 * w_b ** 0.75)
+    p_c = 0.25 * (5.0 - math.sqrt(1.0 + 24.0 * f_c))
+    p_cb = 0.25 * (5.0 - math.sqrt(1.0 + 24.0 * f_cb))
 
-    def sig0(self, R):
-        """S(R) by the reference's rule (sigmas.py:176-191, 224-232)."""
-        R = float(R)
-        s = self._s0.get(R)
-        if s is None:
-            s = self._s0[R] = sig0_trapz(R, self.del2)
-        return s
+    ok = 1.0 - om - ol  # curvature
+    e2 = ol + (ok + om)  # at z = 0
+    om_z, ol_z = om / e2, ol / e2
+    g0 = one_z_eq * 2.5 * om_z / (om_z ** (4.0 / 7.0) - ol_z + (1.0 + om_z / 2.0) * (1.0 + ol_z / 70.0))
+
+    a_nu = (f_c / f_cb * (5.0 - 2.0 * (p_c + p_cb)) / (5.0 - 4.0 * p_cb) * (1.0 + y_d) ** (p_cb - p_c)
+            * (1.0 + f_nub * (-0.553 + 0.126 * f_nub * f_nub))
+            / (1.0 - 0.193 * math.sqrt(f_nu * n_nu) + 0.169 * f_nu * n_nu ** 0.2)
+            * (1.0 + 0.5 * (p_c - p_cb) * (1.0 + 1.0 / ((3.0 - 4.0 * p_c) * (7.0 - 4.0 * p_cb))) / (1.0 + y_d)))
+    sqrt_a = math.sqrt(a_nu)
+    beta_c = 1.0 / (1.0 - 0.949 * f_nub)
+
+    k = np.asarray(k, dtype=np.float64)
+    q = k * theta2 / w_m
+    y_fs = 17.2 * f_nu * (1.0 + 0.488 * f_nu ** (-7.0 / 6.0)) * (n_nu * q / f_nu) ** 2
+    growth_cb = (1.0 + (g0 / (1.0 + y_fs)) ** 0.7) ** (p_cb / 0.7) * g0 ** (1.0 - p_cb)
+    gamma_eff = w_m * (sqrt_a + (1.0 - sqrt_a) / (1.0 + (0.43 * k * s) ** 4))
+    q_eff = k * theta2 / gamma_eff
+    L = np.log(2.71828 + 1.84 * beta_c * sqrt_a * q_eff)
+    C = 14.4 + 325.0 / (1.0 + 60.5 * q_eff ** 1.11)
+    t_sup = L / (L + C * q_eff * q_eff)
+    q_nu = 3.92 * q * math.sqrt(n_nu / f_nu)
+    master = 1.0 + 1.2 * f_nu ** 0.64 * n_nu ** (0.3 + 0.6 * f_nu) / (q_nu ** -1.6 + q_nu ** 0.8)
+    return t_sup * master * growth_cb / g0
 
 
-def eh_nowiggle_transfer(k, cosmo=None, t_cmb=2.728):
+def eh_nowiggle_transfer(k, cosmo=None, t_cmb=T_CMB):
     """Eisenstein & Hu (1998) eqs. 29-31: the transfer function without
     baryon oscillations, with their suppression (alpha_Gamma, sound horizon
     fit); k in Mpc^-1."""
@@ -92,24 +137,54 @@ def eh_nowiggle_transfer(k, cosmo=None, t_cmb=2.728):
     return L / (L + C * q * q)
 
 
-class EHSpectrum:
-    """Delta^2(k) = A k^(3+n) T(k)^2 with the no-wiggle transfer function,
-    A set so that sigma(8/h Mpc) = sigma_8 by the same trapezoid rule."""
+def sigma8_amplitude(shape_fn, cosmo=None):
+    """A such that A * shape(k) (a Delta^2(k) shape) has sigma(8/h Mpc) =
+    sigma_8, by adaptive quadrature over ln k."""
+    c = cosmo_params(cosmo)
+    R8 = 8.0 / c["h"]
 
-    def __init__(self, cosmo=None):
-        c = cosmo_params(cosmo)
-        self.cosmo = c
-        self.n = c["n"]
-        self.amp = 1.0
-        s8 = sig0_trapz(8.0 / c["h"], self.del2)
-        self.amp = c["sigma_8"] ** 2 / s8
-        self._s0 = {}  # S(R) by radius, once the amplitude is set
+    def f(lk):
+        kk = math.exp(lk)
+        return float(shape_fn(np.array([kk]))[0] * tophat_window(np.array([kk * R8]))[0] ** 2)
+
+    I, _ = integrate.quad(f, math.log(1e-7), math.log(1e3), limit=4000, epsabs=0.0, epsrel=1e-11)
+    return c["sigma_8"] ** 2 / I
+
+
+class _Spectrum:
+    """Delta^2(k) = amp * shape(k) with the reference's S(R) rule on top."""
+
+    def _init(self, cosmo):
+        self.cosmo = cosmo_params(cosmo)
+        self.n = float(self.cosmo["n"])
+        self.amp = sigma8_amplitude(self.del2_shape, self.cosmo)
+        self._ktab = np.exp(REFERENCE_LOGK)
+        self._dtab = self.del2_exact(self._ktab)
+        self._s0 = {}  # S(R) by radius
+
+    def del2_exact(self, k):
+        return self.amp * self.del2_shape(k)
 
     def del2(self, k):
+        """Delta^2 tabulated on the reference's k grid and interpolated
+        linearly in k, as interp1d at cosmo_files/sigmas.py:50-52."""
+        return np.interp(np.asarray(k, dtype=np.float64), self._ktab, self._dtab)
+
+    def pk(self, k):
+        """P(k) in Mpc^3 at z = 0 (k in Mpc^-1), 2 pi^2 Delta^2 / k^3."""
         k = np.asarray(k, dtype=np.float64)
-        return self.amp * k ** (3.0 + self.n) * eh_nowiggle_transfer(k, self.cosmo) ** 2
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return np.where(k > 0, 2.0 * math.pi ** 2 * self.del2_exact(k) / np.where(k > 0, k, 1.0) ** 3, 0.0)
+
+    def init_table(self, DIM, box_len, n=512):
+        """(K, P) float32 tables of initialize.py:27-31: n points log-spaced
+        from DELTA_K/10 to DELTA_K sqrt(3) DIM, DELTA_K = float32(2 pi/BOX_LEN)."""
+        dk = float(np.float32(2.0 * np.pi / float(np.float32(box_len))))
+        K = np.logspace(np.log10(dk / 10.0), np.log10(dk * np.sqrt(3.0) * int(DIM)), int(n))
+        return K.astype(np.float32), self.pk(K).astype(np.float32)
 
     def sig0(self, R):
+        """S(R) by the reference's rule (sigmas.py:176-191, 224-232)."""
         R = float(R)
         s = self._s0.get(R)
         if s is None:
@@ -117,18 +192,43 @@ class EHSpectrum:
         return s
 
 
-_TABLE_SPECTRUM = None
+class EH99Spectrum(_Spectrum):
+    """Planck13 Delta^2(k) = A k^(3+n) T(k)^2 with the Eisenstein & Hu (1999)
+    transfer function, sigma_8-normalised (the spectrum of initialize.py:30
+    and of the reference's cached S(R) table)."""
+
+    def __init__(self, cosmo=None):
+        self.cosmo = cosmo_params(cosmo)
+        self._init(self.cosmo)
+
+    def del2_shape(self, k):
+        k = np.asarray(k, dtype=np.float64)
+        return k ** (3.0 + float(self.cosmo["n"])) * eh99_transfer(k, self.cosmo) ** 2
 
 
-def planck_spectrum(kind="table"):
-    global _TABLE_SPECTRUM
-    if kind == "table":
-        if _TABLE_SPECTRUM is None:
-            _TABLE_SPECTRUM = TableSpectrum()
-        return _TABLE_SPECTRUM
-    if kind == "eh_nowiggle":
-        return EHSpectrum()
-    raise ValueError(f"unknown spectrum {kind!r}")
+class EHSpectrum(_Spectrum):
+    """Delta^2(k) = A k^(3+n) T(k)^2 with the Eisenstein & Hu (1998)
+    no-wiggle transfer function, sigma_8-normalised."""
+
+    def __init__(self, cosmo=None):
+        self.cosmo = cosmo_params(cosmo)
+        self._init(self.cosmo)
+
+    def del2_shape(self, k):
+        k = np.asarray(k, dtype=np.float64)
+        return k ** (3.0 + float(self.cosmo["n"])) * eh_nowiggle_transfer(k, self.cosmo) ** 2
+
+
+_SPECTRA = {}
+
+
+def planck_spectrum(kind="eh99"):
+    """The Planck13 spectrum object (cached): "eh99" (default) or "eh_nowiggle"."""
+    if kind not in ("eh99", "eh_nowiggle"):
+        raise ValueError(f"unknown spectrum {kind!r}")
+    if kind not in _SPECTRA:
+        _SPECTRA[kind] = EH99Spectrum() if kind == "eh99" else EHSpectrum()
+    return _SPECTRA[kind]
 
 
 def sigma_r(R, spectrum=None):
@@ -252,6 +352,7 @@ def sigma_8(spectrum=None, cosmo=None):
     return sigma_r(8.0 / c["h"], spectrum)
 
 
-__all__ = ["TableSpectrum", "EHSpectrum", "eh_nowiggle_transfer", "planck_spectrum", "sigma_r", "fgrowth", "dicke",
+__all__ = ["EH99Spectrum", "EHSpectrum", "eh99_transfer", "eh_nowiggle_transfer", "sigma8_amplitude",
+           "planck_spectrum", "sigma_r", "fgrowth", "dicke",
            "hubble", "ddt_growth_over_growth", "virial_mass", "mass_to_radius", "sig_del", "collapse_fraction",
            "barrier", "sigma_8"]

@@ -582,17 +582,33 @@ int run_group(const Prepared& p, size_t i, const cm21_bubble_params* prm, void* 
 // Groups are independent: the first runs on the caller's stream, the others
 // on library helper streams forked after the table upload and joined back at
 // the end, so the small passes of one group overlap the persistent passes of
-// another.
+// another.  Groups that share a plan (same D and box count, different
+// box_len) share its rocFFT work buffer, so they go to the same stream and
+// their forward transforms run one after the other.
 int run_all(const Prepared& p, const cm21_bubble_params* prm, void* work, hipStream_t st) {
     const int G = (int)p.lay.groups.size();
     if (G == 1) return run_group(p, 0, prm, work, st);
     SideStreams* ss = nullptr;
     CM21_TRY(side_streams(&ss));
     std::lock_guard<std::mutex> lk(ss->mu);
-    const int nside = std::min(G - 1, SideStreams::kStreams);
+    // lane[i]: 0 = the caller's stream, k >= 1 = helper stream k-1
+    std::vector<int> lane(G, 0);
+    int next = 0;
+    for (int i = 1; i < G; ++i) {
+        int shared = -1;
+        for (int j = 0; j < i && shared < 0; ++j)
+            if (p.lay.groups[j].g->plan == p.lay.groups[i].g->plan) shared = j;
+        if (shared >= 0) {
+            lane[i] = lane[shared];
+        } else {
+            lane[i] = 1 + next % SideStreams::kStreams;
+            ++next;
+        }
+    }
+    const int nside = std::min(next, SideStreams::kStreams);
     CM21_HIP_CHECK(hipEventRecord(ss->fork, st));
     for (int k = 0; k < nside; ++k) CM21_HIP_CHECK(hipStreamWaitEvent(ss->s[k], ss->fork, 0));
-    for (int i = 0; i < G; ++i) CM21_TRY(run_group(p, i, prm, work, i == 0 ? st : ss->s[(i - 1) % nside]));
+    for (int i = 0; i < G; ++i) CM21_TRY(run_group(p, i, prm, work, lane[i] == 0 ? st : ss->s[lane[i] - 1]));
     for (int k = 0; k < nside; ++k) {
         CM21_HIP_CHECK(hipEventRecord(ss->join[k], ss->s[k]));
         CM21_HIP_CHECK(hipStreamWaitEvent(st, ss->join[k], 0));

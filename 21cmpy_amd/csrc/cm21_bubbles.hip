@@ -245,6 +245,7 @@ struct LoopWork {
     void* sphere;
     unsigned long long* mask;
     double* lpart;
+    float* bmax;
     size_t bytes;
 };
 
@@ -272,6 +273,8 @@ LoopWork carve(void* base, int D, int n_radii, int update) {
     off = align_up(off + sizeof(unsigned long long) * ((N + 63) / 64), 256);
     w.lpart = (double*)(p + off);
     off = align_up(off + sizeof(double) * (fused_partials(D) + 1), 256);
+    w.bmax = (float*)(p + off);
+    off = align_up(off + sizeof(float) * (fused_partials(D) + 1), 256);
     w.bytes = off;
     return w;
 }
@@ -359,18 +362,22 @@ int run_loop(cm21_plan* plan, const float* delta, float* ionized, const float* r
             CM21_HIP_CHECK(hipEventRecord(rows_done[slot], ss));
             CM21_HIP_CHECK(hipStreamWaitEvent(st, rows_done[slot], 0));
             // Nonlinear input: a sum pass for the FP64 box mean, then the
-            // threshold pass recomputes the (cheap) last-axis transform rather
-            // than storing and re-reading the smoothed field.
+            // threshold pass recomputes the last-axis transform (only in the
+            // blocks that can change) rather than storing and re-reading the
+            // smoothed field.
+            // The sum pass also records each block's maximum of the smoothed
+            // field, which lets the threshold pass skip blocks that cannot
+            // reach the threshold.
             const double* sum = nullptr;
             if (!prm->linear_input) {
                 LineEpilogue es = line_epilogue(prm->deltac, denoms[r], prm->zeta, nullptr, prm->fc_mean, w.mask,
-                                                ionized, w.lpart, (double)N);
+                                                ionized, w.lpart, (double)N, w.bmax);
                 CM21_TRY(fused_lines(*sh, 0, T[slot], tw, fp, es, st));
                 CM21_TRY(fused_sum(w.lpart, (int)fused_partials(D), w.sums + r, st));
                 sum = w.sums + r;
             }
             LineEpilogue ep = line_epilogue(prm->deltac, denoms[r], prm->zeta, sum, prm->fc_mean, w.mask, ionized,
-                                            w.lpart, (double)N);
+                                            w.lpart, (double)N, prm->linear_input ? nullptr : w.bmax);
             CM21_TRY(fused_lines(*sh, final_step ? 2 : 1, T[slot], tw, fp, ep, st));
             CM21_HIP_CHECK(hipEventRecord(lines_done[slot], st));
         }

@@ -47,11 +47,17 @@ __global__ __launch_bounds__(W* D1) void k_fused_rows(float2* __restrict__ T, co
 }
 
 // Pass 3: c2r along axis 2 for L consecutive lines, then the f_coll epilogue:
-// FP64 block partial of the box sum (MODE kLineSum), threshold into the
-// ionization bitmask (kLineThreshold) or the final paint (kLineFinal).  In
-// sum mode a one-block kernel adds the partials in a fixed order
-// (deterministic), and the kernel boundary makes them visible across the
-// chip's L2 slices.
+// FP64 block partial of the box sum and the block's maximum of the smoothed
+// field (MODE kLineSum), threshold into the ionization bitmask
+// (kLineThreshold) or the final paint (kLineFinal).  In sum mode a one-block
+// kernel adds the partials in a fixed order (deterministic), and the kernel
+// boundary makes them visible across the chip's L2 slices.
+//
+// Before any transform work a threshold block returns when it cannot change
+// the mask: every one of its cells is already ionized, or the box-sum pass
+// found its maximum below d* - band (no cell can reach the threshold; cells
+// inside the band take the exact test, so the skip never drops a hit).  A
+// final-paint block whose cells are all ionized writes 1 without transforming.
 template <int N1, int N2, int L, int MODE>
 __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict__ U, const float2* __restrict__ tw,
                                                       FusedParams fp, LineEpilogue ep) {
@@ -60,28 +66,37 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
     __shared__ float2 buf[L * C::HP];
     __shared__ float2 stw[D];
     __shared__ unsigned long long smask[WORDS];
+    __shared__ DensityThreshold sth;
     const int t = threadIdx.x;
     const size_t line0 = (size_t)blockIdx.x * L;
     const size_t word0 = line0 * D / 64;
+    const size_t cell0 = line0 * D;
+    const bool use_scale = ep.sum != nullptr;
+    float scale = 1.0f;
+    unsigned long long mword = 0ull;
+    if (MODE != kLineSum) {
+        if (use_scale) scale = (float)((double)ep.fc_mean / (*ep.sum / ep.ncells));
+        if (t == 0) sth = density_threshold(ep.deltac, ep.denom, ep.zeta, scale);
+        mword = t < WORDS ? ep.mask[word0 + t] : ~0ull;
+        const bool all_ionized = __syncthreads_and(mword == ~0ull);
+        if (all_ionized) {
+            if (MODE == kLineFinal)
+                for (int q = t; q < L * D; q += NT) ep.out[cell0 + q] = 1.0f;
+            return;
+        }
+        if (MODE == kLineThreshold && ep.bmax != nullptr && ep.bmax[blockIdx.x] < sth.dstar - sth.band) return;
+    }
     // all global loads in flight before the first LDS store
     float2 tmp[C::LOADS];
     C::load(U + line0 * H, H, fp.cmax + 1, tmp);
-    unsigned long long mword = 0ull;
-    if (MODE != kLineSum && t < WORDS) mword = ep.mask[word0 + t];
     for (int q = t; q < D; q += NT) stw[q] = tw[q];
     C::store(tmp, buf);
     if (MODE != kLineSum && t < WORDS) smask[t] = mword;
     __syncthreads();
     C::transform(buf, stw);
-    const size_t cell0 = line0 * D;
-    const bool use_scale = ep.sum != nullptr;
-    float scale = 1.0f;
-    if (MODE != kLineSum && use_scale) scale = (float)((double)ep.fc_mean / (*ep.sum / ep.ncells));
-    __shared__ DensityThreshold sth;
-    if (MODE != kLineSum && t == 0) sth = density_threshold(ep.deltac, ep.denom, ep.zeta, scale);
-    __syncthreads();
     const DensityThreshold th = sth;
     double acc = 0.0;
+    float vmax = -INFINITY;
     const int lane = t & 63;
     const float* xrf = reinterpret_cast<const float*>(buf);
     const float inv_denom = 1.0f / ep.denom;
@@ -91,6 +106,7 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
         const float delta = xrf[i * 2 * C::HP + r];
         if (MODE == kLineSum) {
             acc += (double)fcoll_for_sum(delta, ep.deltac, inv_denom);
+            vmax = fmaxf(vmax, delta);
         } else {
             const bool hit = cell_hit(delta, th, ep.deltac, ep.denom, use_scale, scale, ep.inv_zeta);
             const unsigned long long word = smask[f >> 6];
@@ -106,6 +122,18 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
         }
     }
     if (MODE == kLineSum) {
+        if (ep.bmax != nullptr) {
+            __shared__ float wmax[NT / 64];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
+            if (lane == 0) wmax[t >> 6] = vmax;
+            __syncthreads();
+            if (t == 0) {
+                float m = wmax[0];
+                for (int w = 1; w < NT / 64; ++w) m = fmaxf(m, wmax[w]);
+                ep.bmax[blockIdx.x] = m;
+            }
+        }
         acc = block_sum(acc);
         if (t == 0) ep.partial[blockIdx.x] = acc;
     }
@@ -263,8 +291,9 @@ int fused_mask_clear(unsigned long long* mask, size_t words, hipStream_t st) {
 }
 
 LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double* sum, float fc_mean,
-                           unsigned long long* mask, float* out, double* partial, double ncells) {
+                           unsigned long long* mask, float* out, double* partial, double ncells, float* bmax) {
     LineEpilogue ep{};
+    ep.bmax = bmax;
     ep.deltac = deltac;
     ep.denom = denom;
     ep.zeta = zeta;

@@ -26,6 +26,9 @@ struct LineEpilogue {
     float* out;                // final float box
     double* partial;           // per-block FP64 partial sums
     double ncells;
+    float* bmax;               // per-block maximum of the smoothed field: written by
+                               // the box-sum pass, read by the threshold pass to skip
+                               // blocks no cell of which can reach the threshold; or NULL
 };
 
 struct FusedShape {
@@ -40,7 +43,8 @@ const FusedShape* fused_shape(int D);
 size_t fused_partials(int D);
 FusedParams fused_params(int D, float dk, float R, int filter_type, int ref_fallthrough, float scale);
 LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double* sum, float fc_mean,
-                           unsigned long long* mask, float* out, double* partial, double ncells);
+                           unsigned long long* mask, float* out, double* partial, double ncells,
+                           float* bmax = nullptr);
 int fused_radius(const FusedShape& sh, const float2* S, float2* T, const float2* tw, const FusedParams& fp,
                  hipStream_t st);
 int fused_lines(const FusedShape& sh, int mode, const float2* U, const float2* tw, const FusedParams& fp,

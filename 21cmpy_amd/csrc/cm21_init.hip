@@ -48,9 +48,40 @@ struct GrfParams {
     double k_cut;    // Gaussian cut-off scale
     double volume;   // box_len^3
     uint32_t seed, box_id;
+    const float* ktab;  // tabulated P(k): increasing K (NULL = power law)
+    const float* ptab;
+    int ntab;
 };
 
+constexpr int kMaxTable = 2048;
+
+// P(k) from the table by the rule of initialize.cu:87-90: i is the first
+// index with K[i] >= k and P is interpolated linearly between i-1 and i
+// (in FP64 here); modes outside [K[0], K[n-1]] get 0.
+__device__ inline double table_pk(double k, const float* K, const float* Pt, int n) {
+    if (!(k >= (double)K[0]) || k > (double)K[n - 1]) return 0.0;
+    int lo = 0, hi = n - 1;  // K[lo] <= k <= K[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((double)K[mid] >= k)
+            hi = mid;
+        else
+            lo = mid;
+    }
+    if ((double)K[lo] >= k) return (double)Pt[lo];  // k == K[0]
+    const double k0 = K[lo], k1 = K[hi], p0 = Pt[lo], p1 = Pt[hi];
+    return p0 + (p1 - p0) * (k - k0) / (k1 - k0);
+}
+
 __global__ void k_grf(float2* __restrict__ spec, int D, size_t Nc, GrfParams gp) {
+    __shared__ float sK[kMaxTable], sP[kMaxTable];
+    if (gp.ntab > 0) {
+        for (int i = threadIdx.x; i < gp.ntab; i += blockDim.x) {
+            sK[i] = gp.ktab[i];
+            sP[i] = gp.ptab[i];
+        }
+        __syncthreads();
+    }
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= Nc) return;
     const int H = D / 2 + 1;
@@ -78,8 +109,13 @@ __global__ void k_grf(float2* __restrict__ spec, int D, size_t Nc, GrfParams gp)
     const double na = mode_of(a, D), nb = mode_of(b, D), nc = c;
     const double m2 = na * na + nb * nb + nc * nc;
     const double k = gp.dk * sqrt(m2);
-    const double x = k / gp.k_cut;
-    const double P = gp.amp * (pow(k, gp.n_index) * exp(-(x * x)));
+    double P;
+    if (gp.ntab > 0) {
+        P = table_pk(k, sK, sP, gp.ntab);
+    } else {
+        const double x = k / gp.k_cut;
+        P = gp.amp * (pow(k, gp.n_index) * exp(-(x * x)));
+    }
     const double amp = sqrt(P / gp.volume);
     double re, im;
     if (selfconj) {
@@ -151,7 +187,24 @@ extern "C" int cm21_grf_init(void* spec_out, int D, double box_len, double amp, 
     CM21_REQUIRE(valid_dim(D), "cm21_grf_init: D=%d must be even in [2,2048]", D);
     CM21_REQUIRE(box_len > 0.0 && k_cut > 0.0 && amp >= 0.0, "cm21_grf_init: box_len, k_cut must be > 0, amp >= 0");
     const size_t Nc = (size_t)D * D * (D / 2 + 1);
-    GrfParams gp{2.0 * M_PI / box_len, amp, n_index, k_cut, box_len * box_len * box_len, seed, box_id};
+    GrfParams gp{2.0 * M_PI / box_len, amp, n_index, k_cut, box_len * box_len * box_len, seed, box_id,
+                 nullptr, nullptr, 0};
+    k_grf<<<grid_for(Nc), kThreads, 0, (hipStream_t)stream>>>((float2*)spec_out, D, Nc, gp);
+    CM21_LAUNCH_CHECK("k_grf");
+    return CM21_OK;
+}
+
+extern "C" int cm21_grf_init_table(void* spec_out, int D, double box_len, const float* k_table,
+                                   const float* p_table, int n_table, uint32_t seed, uint32_t box_id,
+                                   void* stream) {
+    CM21_REQUIRE(spec_out && k_table && p_table, "cm21_grf_init_table: NULL argument");
+    CM21_REQUIRE(valid_dim(D), "cm21_grf_init_table: D=%d must be even in [2,2048]", D);
+    CM21_REQUIRE(box_len > 0.0, "cm21_grf_init_table: box_len must be > 0");
+    CM21_REQUIRE(n_table >= 2 && n_table <= kMaxTable, "cm21_grf_init_table: n_table=%d not in [2,%d]", n_table,
+                 kMaxTable);
+    const size_t Nc = (size_t)D * D * (D / 2 + 1);
+    GrfParams gp{2.0 * M_PI / box_len, 0.0, 0.0, 1.0, box_len * box_len * box_len, seed, box_id,
+                 k_table, p_table, n_table};
     k_grf<<<grid_for(Nc), kThreads, 0, (hipStream_t)stream>>>((float2*)spec_out, D, Nc, gp);
     CM21_LAUNCH_CHECK("k_grf");
     return CM21_OK;

@@ -111,6 +111,23 @@ def grf_init(D, box_len, amp, seed, box_id=0, n_index=-2.5, k_cut=None, out=None
     return out
 
 
+def grf_init_table(D, box_len, k_table, p_table, seed, box_id=0, out=None):
+    """GRF half spectrum with P(k) interpolated from a (K, P) table
+    (float32, K increasing), as init_kernel does (initialize.cu:58-95)."""
+    out = empty_spec(D) if out is None else out
+    _check_spec(out, D, "out")
+    K = torch.as_tensor(np.ascontiguousarray(k_table, dtype=np.float32)).to(_DEV)
+    Pt = torch.as_tensor(np.ascontiguousarray(p_table, dtype=np.float32)).to(_DEV)
+    if K.ndim != 1 or K.shape != Pt.shape or not 2 <= K.numel() <= 2048:
+        raise ValueError("k_table and p_table must be 1-D float32 arrays of equal length in [2, 2048]")
+    if not bool(torch.all(K[1:] > K[:-1])):
+        raise ValueError("k_table must be strictly increasing")
+    _lib.call("cm21_grf_init_table", _lib.ptr(out), int(D), float(box_len), _lib.ptr(K), _lib.ptr(Pt), int(K.numel()),
+              ctypes.c_uint32(int(seed) & 0xFFFFFFFF), ctypes.c_uint32(int(box_id) & 0xFFFFFFFF),
+              _lib.stream_handle())
+    return out
+
+
 def subsample(large, sw, out=None):
     D = large.shape[0]
     _check_box(large, D, "large")

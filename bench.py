@@ -336,10 +336,10 @@ def main():
         return r, (time.perf_counter() - t) * 1e3
 
     # pipeline stages (once; the first call of each includes plan creation)
-    ics, _ = timed(lambda: init.init(DIM=DIM, HII_DIM=D, BOX_LEN=L, seed=seed, save=False))
-    ics, t_init = timed(lambda: init.init(DIM=DIM, HII_DIM=D, BOX_LEN=L, seed=seed, save=False))
-    upd, _ = timed(lambda: pert.evolve_zeldovich(8.0, ics["deltax"], ics["v"], growth, growth_init, L, "cic"))
-    upd, t_pert = timed(lambda: pert.evolve_zeldovich(8.0, ics["deltax"], ics["v"], growth, growth_init, L, "cic"))
+    ics, _ = timed(lambda: init.init(DIM=DIM, HII_DIM=D, BOX_LEN=L, seed=seed, pspec="powerlaw", save=False))
+    ics, t_init = timed(lambda: init.init(DIM=DIM, HII_DIM=D, BOX_LEN=L, seed=seed, pspec="powerlaw", save=False))
+    upd, _ = timed(lambda: pert.evolve_zeldovich(8.0, ics["deltax"], ics["v"], growth, growth_init, L, "cic", save=False))
+    upd, t_pert = timed(lambda: pert.evolve_zeldovich(8.0, ics["deltax"], ics["v"], growth, growth_init, L, "cic", save=False))
     del ics
     torch.cuda.empty_cache()
 

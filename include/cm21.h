@@ -151,6 +151,15 @@ int cm21_grf_init(void* spec_out, int D, double box_len, double amp,
                   double n_index, double k_cut, uint32_t seed, uint32_t box_id,
                   void* stream);
 
+/* Replaces init_kernel with its tabulated spectrum (Programs/initialize.cu:
+ * 58-95; table of initialize.py:27-31): as cm21_grf_init, with P(k) (Mpc^3)
+ * interpolated from n_table (K, P) pairs in device memory (float32, K
+ * increasing, 2 <= n_table <= 2048): i is the first index with K[i] >= k and
+ * P = P[i-1] + (P[i]-P[i-1]) (k-K[i-1])/(K[i]-K[i-1]); modes outside
+ * [K[0], K[n_table-1]] get 0.  k = 2 pi |n| / box_len in FP64. */
+int cm21_grf_init_table(void* spec_out, int D, double box_len, const float* k_table, const float* p_table,
+                        int n_table, uint32_t seed, uint32_t box_id, void* stream);
+
 /* Replaces subsample (Programs/initialize.cu:133-146): small[i,j,k] =
  * large[l(i),l(j),l(k)], l(i) = floor(float32(i*pixel_factor) + 0.5). */
 int cm21_subsample(float* small, const float* large, int D, int sw,

@@ -22,8 +22,19 @@ from .sigma import mode_index, synthetic_amplitude, synthetic_pk_shape
 from .bubbles import kmag_half_f32, window_f32, apply_window_f32, delta_k_f32
 
 
-def grf_half_spectrum(D, box_len, seed, box_id=0, sigma_cell2=0.36):
-    """complex64 (D, D, D//2+1) c2r-ready Hermitian GRF coefficients (FP64 math)."""
+def table_pk(k, k_table, p_table):
+    """Linear interpolation of a float32 (K, P) table, initialize.cu:87-90
+    (i = first index with K[i] >= k; 0 outside the table), in FP64."""
+    K = np.asarray(k_table, dtype=np.float32).astype(np.float64)
+    Pt = np.asarray(p_table, dtype=np.float32).astype(np.float64)
+    return np.interp(np.asarray(k, dtype=np.float64), K, Pt, left=0.0, right=0.0)
+
+
+def grf_half_spectrum(D, box_len, seed, box_id=0, sigma_cell2=0.36, pk_table=None):
+    """complex64 (D, D, D//2+1) c2r-ready Hermitian GRF coefficients (FP64 math).
+
+    pk_table: None for the synthetic power law of the benchmark recipe, or a
+    float32 (K, P) pair interpolated as the reference's init_kernel does."""
     H = D // 2 + 1
     a = np.arange(D)[:, None, None]
     b = np.arange(D)[None, :, None]
@@ -50,9 +61,12 @@ def grf_half_spectrum(D, box_len, seed, box_id=0, sigma_cell2=0.36):
     n = mode_index(D).astype(np.float64)
     m2 = n[a] ** 2 + n[b] ** 2 + c.astype(np.float64) ** 2
     k = dk * np.sqrt(m2)
-    A = synthetic_amplitude(D, box_len, sigma_cell2)
     V = float(box_len) ** 3
-    amp = np.sqrt(A * synthetic_pk_shape(k, np.pi * D / float(box_len)) / V)
+    if pk_table is None:
+        A = synthetic_amplitude(D, box_len, sigma_cell2)
+        amp = np.sqrt(A * synthetic_pk_shape(k, np.pi * D / float(box_len)) / V)
+    else:
+        amp = np.sqrt(table_pk(k, *pk_table) / V)
 
     selfconj = edge & (ma == a) & (mb == b)
     re = np.where(selfconj, amp * g1, amp * g1 / np.sqrt(2.0))
@@ -108,7 +122,7 @@ def velocity_multiplier(D, box_len, comp):
     return 1j * np.broadcast_to(f, (D, D, D // 2 + 1))
 
 
-def initialize(D, hii_dim, box_len, seed, box_id=0, sigma_cell2=0.36, with_velocity=True):
+def initialize(D, hii_dim, box_len, seed, box_id=0, sigma_cell2=0.36, with_velocity=True, pk_table=None):
     """Initial conditions as initialize.init (initialize.py:33-111).
 
     Returns dict with ``deltax`` (D^3), ``smoothed`` (hii_dim^3) and, when asked,
@@ -117,7 +131,7 @@ def initialize(D, hii_dim, box_len, seed, box_id=0, sigma_cell2=0.36, with_veloc
     initialize.cu:24; the windows and the final casts are float32, the
     transforms FP64.
     """
-    spec = grf_half_spectrum(D, box_len, seed, box_id, sigma_cell2)
+    spec = grf_half_spectrum(D, box_len, seed, box_id, sigma_cell2, pk_table)
     out = {"spec": spec, "deltax": c2r(spec, D).astype(np.float32)}
     R = smoothing_radius(box_len, hii_dim)
     kmag = kmag_half_f32(D, delta_k_f32(box_len), fastest_first=False)

@@ -126,14 +126,16 @@ def synthetic_del2_fn(D, box_len, sigma_cell2=0.36):
 
 # ---------------------------------------------------------------- radius schedule
 
-def radius_schedule(box_len, dim, py2_int_scale=False):
+def radius_schedule(box_len, dim, py2_int_scale=False, scale=None):
     """float32 radii from R_BUBBLE_MAX down, and the cell size used by the stop rule.
 
-    find_HII_bubbles.py:53-54 (scale = BoxSize/HIIdim, integer division under
-    Python 2 when both are ints), :92 (R = R_BUBBLE_MAX), :123-125 (final_step
-    test done in float64 before the float32 cast), :189 (R = R/1.1 in float64).
+    find_HII_bubbles.py:52-54 (a caller's scale, else scale = BoxSize/HIIdim,
+    integer division under Python 2 when both are ints), :92 (R =
+    R_BUBBLE_MAX), :123-125 (final_step test done in float64 before the
+    float32 cast), :189 (R = R/1.1 in float64).
     """
-    scale = (int(box_len) // int(dim)) if py2_int_scale else float(box_len) / float(dim)
+    if scale is None:
+        scale = (int(box_len) // int(dim)) if py2_int_scale else float(box_len) / float(dim)
     stop = float(L_FACTOR) * scale
     R = float(R_BUBBLE_MAX)
     radii = []

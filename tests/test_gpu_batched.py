@@ -206,3 +206,32 @@ def test_batched_errors(gpu):
                                graph=False, tiles=None)
         bb._garr[0].tile_lines = 99
         bb.run()
+
+
+def test_batched_groups_sharing_a_plan(gpu):
+    """Two groups with the same D and box count but different box_len share
+    one rocFFT plan (and its work buffer); they must not run their forward
+    transforms at the same time.  Each box equals the single-box loop, and a
+    run on a stream of its own, with the boxes loaded on the caller's stream,
+    gives the same result."""
+    bm = pkg("Programs.find_HII_bubbles_batched")
+    specs = _with_sig0([
+        {"D": 64, "box_len": 100.0, "zeta": 30.0, "deltac": 1.686, "sigma_min2": 0.5, "seed": 21},
+        {"D": 64, "box_len": 80.0, "zeta": 25.0, "deltac": 1.686, "sigma_min2": 0.6, "seed": 22},
+        {"D": 64, "box_len": 100.0, "zeta": 40.0, "deltac": 1.7, "sigma_min2": 0.55, "seed": 23},
+        {"D": 64, "box_len": 80.0, "zeta": 20.0, "deltac": 1.65, "sigma_min2": 0.65, "seed": 24},
+    ])
+    deltas = [_box(s["D"], s["box_len"], s["seed"]) for s in specs]
+    for stream in (None, torch.cuda.Stream()):
+        bb = bm.BatchedBubbles(specs, LE=True, graph=stream is not None, tiles=None, stream=stream)
+        assert [(g[0], len(g[2])) for g in bb.groups] == [(64, 2), (64, 2)]
+        assert bb._groups[0]["plan"] is bb._groups[1]["plan"]
+        bb.set_deltas(deltas)
+        for _ in range(2):
+            bb.run()
+            for i, (s, d) in enumerate(zip(specs, deltas)):
+                got = bb.ionized(i).cpu().numpy()
+                print(f"stream={stream is not None} box {i} L={s['box_len']} x_ion={np.mean(got == 1):.3f}",
+                      flush=True)
+                assert 0.01 < np.mean(got == 1) < 0.99
+                _close_to_single(got, _single(d, s, True))

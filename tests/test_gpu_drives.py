@@ -21,10 +21,10 @@ def test_zscroll_matches_stage_by_stage(gpu):
     out = drv.drive_zscroll_noTs(12.0, 6.0, -2.0, DIM=DIM, HII_DIM=D, BOX_LEN=L, seed=3, zeta=30.0, sigma_min2=40.0,
                                  quiet=True)
     assert list(out["z"]) == [12.0, 10.0, 8.0]
-    ics = init.init(DIM=DIM, HII_DIM=D, BOX_LEN=L, seed=3, save=False)
+    ics = init.init(DIM=DIM, HII_DIM=D, BOX_LEN=L, seed=3, pspec="powerlaw", save=False)
     d_init = bm.linear_growth(300.0)
     for i, z in enumerate(out["z"]):
-        upd = pert.evolve_zeldovich(z, ics["deltax"], ics["v"], bm.linear_growth(z), d_init, L, "cic")
+        upd = pert.evolve_zeldovich(z, ics["deltax"], ics["v"], bm.linear_growth(z), d_init, L, "cic", save=False)
         # the CIC deposit uses float atomics: equal up to the order of additions
         assert torch.allclose(upd, out["density"][i], rtol=0, atol=1e-5)
         single = fb.conv_bubbles(out["density"][i], {"BoxSize": L, "HIIdim": D}, LE=False, quiet=True,

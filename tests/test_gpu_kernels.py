@@ -110,6 +110,41 @@ def test_grf_matches_oracle(gpu):
         assert np.mean(got == want) > 0.999
 
 
+def test_grf_table_matches_oracle(gpu):
+    """cm21_grf_init_table with the Planck13 table of initialize.py:27-31
+    against the oracle's table interpolation (same Philox stream)."""
+    g = pkg("gpu")
+    pl = pkg("cosmo_files.planck")
+    for D, L, seed, box in ((16, 40.0, 3, 0), (64, 128.0, 0, 5)):
+        K, Pk = pl.planck_spectrum().init_table(D, L, 512)
+        got = g.grf_init_table(D, L, K, Pk, seed, box).cpu().numpy()
+        want = grf.grf_half_spectrum(D, L, seed, box, pk_table=(K, Pk))
+        ulp = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64)
+        diff = np.abs(got.astype(np.complex128) - want.astype(np.complex128))
+        assert np.all(diff <= 2 * ulp + 1e-30)
+        assert np.mean(got == want) > 0.999
+    # a short table: modes beyond its last K get no power
+    K = np.array([0.05, 0.2, 0.5], np.float32)
+    Pk = np.array([10.0, 5.0, 1.0], np.float32)
+    got = g.grf_init_table(16, 40.0, K, Pk, 1).cpu().numpy()
+    want = grf.grf_half_spectrum(16, 40.0, 1, pk_table=(K, Pk))
+    assert np.allclose(got, want, rtol=1e-6, atol=1e-12)
+    assert np.any(got == 0) and np.any(got != 0)
+
+
+def test_init_pipeline_planck_table_matches_oracle(gpu):
+    init = pkg("Programs.initialize")
+    D, sw, L, seed = 64, 32, 128.0, 4
+    out = init.init(DIM=D, HII_DIM=sw, BOX_LEN=L, seed=seed, save=False)
+    want = grf.initialize(D, sw, L, seed, pk_table=out["pk_table"])
+    for key in ("deltax", "smoothed"):
+        a = out[key].cpu().numpy()
+        assert np.abs(a - want[key]).max() < 1e-5 * np.abs(want[key]).max(), key
+    for c in range(3):
+        a = out["v"][c].cpu().numpy()
+        assert np.abs(a - want["v"][c]).max() < 1e-5 * np.abs(want["v"][c]).max(), c
+
+
 def test_subsample_and_velocity(gpu):
     g = pkg("gpu")
     D, sw, L = 32, 16, 60.0
@@ -131,7 +166,7 @@ def test_subsample_and_velocity(gpu):
 def test_init_pipeline_matches_oracle(gpu):
     init = pkg("Programs.initialize")
     D, sw, L, seed = 64, 32, 100.0, 9
-    out = init.init(DIM=D, HII_DIM=sw, BOX_LEN=L, seed=seed, save=False)
+    out = init.init(DIM=D, HII_DIM=sw, BOX_LEN=L, seed=seed, pspec="powerlaw", save=False)
     want = grf.initialize(D, sw, L, seed)
     for key in ("deltax", "smoothed"):
         a = out[key].cpu().numpy()

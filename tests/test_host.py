@@ -35,7 +35,7 @@ def cdll():
 def test_header_parse_finds_all_entry_points():
     names = header_functions()
     assert "cm21_find_bubbles" in names and "cm21_power_spectrum" in names
-    assert "cm21_delta_T" in names and len(names) == 44
+    assert "cm21_delta_T" in names and len(names) == len(pkg("_lib").SIGNATURES) >= 45
 
 
 def test_library_exports_every_declared_symbol(cdll):
@@ -149,6 +149,22 @@ def test_sig0_memo_returns_the_computed_values():
     assert fb._default_sig0(256, 300).__self__ is fb._default_sig0(256.0, 300.0).__self__
     tab = pl.planck_spectrum()
     assert tab.sig0(5.0) == tab.sig0(5.0) == pl.sig0_trapz(5.0, tab.del2)
+
+
+def test_radius_schedule_uses_callers_scale():
+    """conv_bubbles' scale argument sets the cell size of the stop rule
+    (find_HII_bubbles.py:52-54,123): the radius count follows the oracle's
+    schedule with the same scale, not BoxSize/HIIdim."""
+    from oracle import sigma
+    fb = pkg("Programs.find_HII_bubbles")
+    for L, D, scale in ((300.0, 256, 3.0), (100.0, 64, 0.5), (200.0, 128, 2.4)):
+        r, cell = fb.radius_schedule(L, D, cell=scale)
+        want, wcell = sigma.radius_schedule(L, D, scale=scale)
+        assert cell == wcell == scale and np.array_equal(r, want)
+        default, _ = fb.radius_schedule(L, D)
+        assert len(r) != len(default)
+        radii, _, _ = fb.bubble_setup(D, L, scale, 1.686, 50.0, lambda R: 0.1)
+        assert np.array_equal(radii, want)
 
 
 def test_conv_bubbles_fails_loudly_without_gpu():
