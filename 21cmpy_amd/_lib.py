@@ -156,7 +156,7 @@ SIGNATURES = {
 }
 
 STAGES = ["r2c", "filter", "c2r", "fcoll_sum", "threshold", "sphere", "fused_columns", "fused_rows",
-          "fused_lines"]
+          "fused_lines", "fused_lines_threshold"]
 
 
 def profile_enable(stages=True):

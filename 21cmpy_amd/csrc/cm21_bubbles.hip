@@ -246,6 +246,7 @@ struct LoopWork {
     unsigned long long* mask;
     double* lpart;
     float* bmax;
+    ScaledThreshold* scaled;
     size_t bytes;
 };
 
@@ -275,6 +276,8 @@ LoopWork carve(void* base, int D, int n_radii, int update) {
     off = align_up(off + sizeof(double) * (fused_partials(D) + 1), 256);
     w.bmax = (float*)(p + off);
     off = align_up(off + sizeof(float) * (fused_partials(D) + 1), 256);
+    w.scaled = (ScaledThreshold*)(p + off);
+    off = align_up(off + sizeof(ScaledThreshold) * (size_t)(n_radii > 0 ? n_radii : 1), 256);
     w.bytes = off;
     return w;
 }
@@ -292,27 +295,35 @@ extern "C" size_t cm21_find_bubbles_workspace_bytes(int D, int n_radii, int upda
     return carve(nullptr, D, n_radii, update).bytes;
 }
 
-extern "C" int cm21_find_bubbles(cm21_plan* plan, const float* delta, float* ionized, const float* radii,
-                                 const float* denoms, int n_radii, const cm21_bubble_params* prm, void* work,
-                                 size_t work_bytes, void* stream) {
-    CM21_REQUIRE(plan && delta && ionized && radii && denoms && prm && work, "cm21_find_bubbles: NULL argument");
-    const int D = plan_dim(plan);
-    CM21_REQUIRE(plan_batch(plan) == 1, "cm21_find_bubbles: plan must have batch 1");
-    CM21_REQUIRE(n_radii >= 1 && n_radii <= 4096, "cm21_find_bubbles: n_radii=%d out of range", n_radii);
-    CM21_REQUIRE(prm->filter_type >= 0 && prm->filter_type <= 2, "cm21_find_bubbles: filter_type=%d",
-                 prm->filter_type);
-    CM21_REQUIRE(prm->zeta > 0.0, "cm21_find_bubbles: zeta must be positive");
-    CM21_REQUIRE(prm->box_len > 0.0f, "cm21_find_bubbles: box_len must be positive");
+namespace {
+// Argument checks shared by cm21_find_bubbles and cm21_find_bubbles_noalloc.
+int check_loop_args(const char* fn, cm21_plan* plan, const float* delta, float* ionized, const float* radii,
+                    const float* denoms, int n_radii, const cm21_bubble_params* prm) {
+    CM21_REQUIRE(plan && delta && ionized && radii && denoms && prm, "%s: NULL argument", fn);
+    CM21_REQUIRE(plan_batch(plan) == 1, "%s: plan must have batch 1", fn);
+    CM21_REQUIRE(n_radii >= 1 && n_radii <= 4096, "%s: n_radii=%d out of range", fn, n_radii);
+    CM21_REQUIRE(prm->filter_type >= 0 && prm->filter_type <= 2, "%s: filter_type=%d", fn, prm->filter_type);
+    CM21_REQUIRE(prm->zeta > 0.0, "%s: zeta must be positive", fn);
+    CM21_REQUIRE(prm->box_len > 0.0f, "%s: box_len must be positive", fn);
     for (int r = 0; r < n_radii; ++r) {
         if (!(denoms[r] > 0.0f) || !std::isfinite(denoms[r]))
             return cm21::fail(CM21_ERR_DOMAIN,
-                              "cm21_find_bubbles: denominator %g at R=%g is not positive and finite "
-                              "(sigma_min^2 <= S(R))",
+                              "%s: denominator %g at R=%g is not positive and finite (sigma_min^2 <= S(R))", fn,
                               (double)denoms[r], (double)radii[r]);
         if (prm->update && !(radii[r] * prm->sphere_r_scale <= 1024.0f))
-            return cm21::fail(CM21_ERR_INVALID, "cm21_find_bubbles: sphere radius %g cells exceeds 1024",
+            return cm21::fail(CM21_ERR_INVALID, "%s: sphere radius %g cells exceeds 1024", fn,
                               (double)(radii[r] * prm->sphere_r_scale));
     }
+    return CM21_OK;
+}
+}  // namespace
+
+extern "C" int cm21_find_bubbles(cm21_plan* plan, const float* delta, float* ionized, const float* radii,
+                                 const float* denoms, int n_radii, const cm21_bubble_params* prm, void* work,
+                                 size_t work_bytes, void* stream) {
+    CM21_TRY(check_loop_args("cm21_find_bubbles", plan, delta, ionized, radii, denoms, n_radii, prm));
+    CM21_REQUIRE(work, "cm21_find_bubbles: NULL argument");
+    const int D = plan_dim(plan);
     LoopWork w = carve(work, D, n_radii, prm->update);
     if (work_bytes < w.bytes)
         return cm21::fail(CM21_ERR_NOMEM, "cm21_find_bubbles: workspace %zu bytes < required %zu", work_bytes,
@@ -368,17 +379,26 @@ int run_loop(cm21_plan* plan, const float* delta, float* ionized, const float* r
             // The sum pass also records each block's maximum of the smoothed
             // field, which lets the threshold pass skip blocks that cannot
             // reach the threshold.
+            // The box-mean scale and the density threshold are derived once
+            // per radius: by the reduction kernel for nonlinear input, on the
+            // host for linear input (scale 1).
             const double* sum = nullptr;
             if (!prm->linear_input) {
                 LineEpilogue es = line_epilogue(prm->deltac, denoms[r], prm->zeta, nullptr, prm->fc_mean, w.mask,
                                                 ionized, w.lpart, (double)N, w.bmax);
                 CM21_TRY(fused_lines(*sh, 0, T[slot], tw, fp, es, st));
-                CM21_TRY(fused_sum(w.lpart, (int)fused_partials(D), w.sums + r, st));
+                CM21_TRY(fused_sum(w.lpart, (int)fused_partials(D), w.sums + r, st, w.scaled + r, prm->fc_mean,
+                                   (double)N, prm->deltac, denoms[r], prm->zeta));
                 sum = w.sums + r;
             }
             LineEpilogue ep = line_epilogue(prm->deltac, denoms[r], prm->zeta, sum, prm->fc_mean, w.mask, ionized,
                                             w.lpart, (double)N, prm->linear_input ? nullptr : w.bmax);
-            CM21_TRY(fused_lines(*sh, final_step ? 2 : 1, T[slot], tw, fp, ep, st));
+            if (prm->linear_input)
+                ep.th = density_threshold_host(prm->deltac, denoms[r], prm->zeta, 1.0f);
+            else
+                ep.scaled = w.scaled + r;
+            CM21_TRY(fused_lines(*sh, final_step ? 2 : 1, T[slot], tw, fp, ep, st,
+                                 prm->linear_input ? CM21_STAGE_FUSED_LINES : CM21_STAGE_FUSED_LINES_THRESHOLD));
             CM21_HIP_CHECK(hipEventRecord(lines_done[slot], st));
         }
         return CM21_OK;

@@ -66,17 +66,20 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
     __shared__ float2 buf[L * C::HP];
     __shared__ float2 stw[D];
     __shared__ unsigned long long smask[WORDS];
-    __shared__ DensityThreshold sth;
     const int t = threadIdx.x;
     const size_t line0 = (size_t)blockIdx.x * L;
     const size_t word0 = line0 * D / 64;
     const size_t cell0 = line0 * D;
-    const bool use_scale = ep.sum != nullptr;
+    const bool use_scale = ep.scaled != nullptr;
     float scale = 1.0f;
+    DensityThreshold th = ep.th;
     unsigned long long mword = 0ull;
     if (MODE != kLineSum) {
-        if (use_scale) scale = (float)((double)ep.fc_mean / (*ep.sum / ep.ncells));
-        if (t == 0) sth = density_threshold(ep.deltac, ep.denom, ep.zeta, scale);
+        if (use_scale) {
+            const ScaledThreshold s = *ep.scaled;
+            scale = s.scale;
+            th = s.th;
+        }
         mword = t < WORDS ? ep.mask[word0 + t] : ~0ull;
         const bool all_ionized = __syncthreads_and(mword == ~0ull);
         if (all_ionized) {
@@ -84,7 +87,7 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
                 for (int q = t; q < L * D; q += NT) ep.out[cell0 + q] = 1.0f;
             return;
         }
-        if (MODE == kLineThreshold && ep.bmax != nullptr && ep.bmax[blockIdx.x] < sth.dstar - sth.band) return;
+        if (MODE == kLineThreshold && ep.bmax != nullptr && ep.bmax[blockIdx.x] < th.dstar - th.band) return;
     }
     // all global loads in flight before the first LDS store
     float2 tmp[C::LOADS];
@@ -94,7 +97,6 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
     if (MODE != kLineSum && t < WORDS) smask[t] = mword;
     __syncthreads();
     C::transform(buf, stw);
-    const DensityThreshold th = sth;
     double acc = 0.0;
     float vmax = -INFINITY;
     const int lane = t & 63;
@@ -153,8 +155,16 @@ __global__ void k_fill_u64(unsigned long long* __restrict__ p, size_t n) {
 // over the 16 wavefronts (deterministic).
 constexpr int kSumThreads = 1024;
 
+// With `scaled` the same thread also derives the radius' box-mean scale and
+// its density threshold, once for every block of the threshold pass.
+struct SumScaleArgs {
+    ScaledThreshold* scaled;
+    float fc_mean, deltac, denom;
+    double ncells, zeta;
+};
+
 __global__ __launch_bounds__(kSumThreads) void k_sum_partials_lines(const double* __restrict__ partial, int n,
-                                                                    double* __restrict__ out) {
+                                                                    double* __restrict__ out, SumScaleArgs sa) {
     __shared__ double part[kSumThreads / 64];
     double acc = 0.0;
     for (int i = threadIdx.x; i < n; i += kSumThreads) acc += partial[i];
@@ -165,6 +175,14 @@ __global__ __launch_bounds__(kSumThreads) void k_sum_partials_lines(const double
         double r = 0.0;
         for (int w = 0; w < kSumThreads / 64; ++w) r += part[w];
         *out = r;
+        if (sa.scaled) {
+            const float scale = (float)((double)sa.fc_mean / (r / sa.ncells));
+            ScaledThreshold s;
+            s.scale = scale;
+            s.th = density_threshold(sa.deltac, sa.denom, sa.zeta, scale);
+            s.pad = 0.0f;
+            *sa.scaled = s;
+        }
     }
 }
 
@@ -277,8 +295,8 @@ int fused_radius(const FusedShape& sh, const float2* S, float2* T, const float2*
 }
 
 int fused_lines(const FusedShape& sh, int mode, const float2* U, const float2* tw, const FusedParams& fp,
-                const LineEpilogue& ep, hipStream_t st) {
-    ProfToken t = prof_begin(CM21_STAGE_FUSED_LINES, st);
+                const LineEpilogue& ep, hipStream_t st, int stage) {
+    ProfToken t = prof_begin(stage, st);
     CM21_TRY(sh.lines(mode, U, tw, fp, ep, st));
     prof_end(t, st);
     return CM21_OK;
@@ -288,6 +306,35 @@ int fused_mask_clear(unsigned long long* mask, size_t words, hipStream_t st) {
     k_fill_u64<<<grid_for(words), kThreads, 0, st>>>(mask, words);
     CM21_LAUNCH_CHECK("k_fill_u64");
     return CM21_OK;
+}
+
+// erfc^-1 on the host: bisection on std::erfc (monotone decreasing) to the
+// last bit, for y in (0, 2).
+static double erfcinv_host(double y) {
+    double lo = -6.0, hi = 27.3;  // erfc(-6) = 2 - 2e-17, erfc(27.3) < 1e-300
+    for (int i = 0; i < 200 && hi - lo > 0.0; ++i) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid == lo || mid == hi) break;
+        if (std::erfc(mid) > y)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+DensityThreshold density_threshold_host(float deltac, float denom, double zeta, float scale) {
+    const double y = 1.0 / (zeta * (double)scale);
+    DensityThreshold t;
+    if (!(y > 1e-30 && y < 1.999)) {
+        t.dstar = 0.0f;
+        t.band = INFINITY;
+        return t;
+    }
+    const double ds = (double)deltac - (double)denom * erfcinv_host(y);
+    t.dstar = (float)ds;
+    t.band = (float)(1e-4 * (1.0 + std::fabs((double)deltac) + (double)denom + std::fabs(ds)));
+    return t;
 }
 
 LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double* sum, float fc_mean,
@@ -307,8 +354,10 @@ LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double*
     return ep;
 }
 
-int fused_sum(const double* partial, int n, double* out, hipStream_t st) {
-    k_sum_partials_lines<<<1, kSumThreads, 0, st>>>(partial, n, out);
+int fused_sum(const double* partial, int n, double* out, hipStream_t st, ScaledThreshold* scaled, float fc_mean,
+              double ncells, float deltac, float denom, double zeta) {
+    SumScaleArgs sa{scaled, fc_mean, deltac, denom, ncells, zeta};
+    k_sum_partials_lines<<<1, kSumThreads, 0, st>>>(partial, n, out, sa);
     CM21_LAUNCH_CHECK("k_sum_partials_lines");
     return CM21_OK;
 }

@@ -17,6 +17,26 @@ struct FusedParams {
     float scale;    // 1/N times the growth factor of nonlinear input
 };
 
+// Density-space form of the centre-pixel threshold: f = erfcf((deltac -
+// d)/denom) * scale >= 1/zeta holds exactly when d >= dstar, up to the float32
+// rounding of the left side; cells within `band` of dstar take the exact
+// float32 test, the others are decided by comparing d with dstar.
+struct DensityThreshold {
+    float dstar;  // decided by d > dstar outside the band
+    float band;   // +inf: every cell takes the exact test
+};
+
+// Box-mean scale of nonlinear input and the threshold it implies, written
+// once per radius by the reduction of the box sum.
+struct ScaledThreshold {
+    float scale;
+    DensityThreshold th;
+    float pad;
+};
+
+// The threshold for linear input (scale 1), computed on the host.
+DensityThreshold density_threshold_host(float deltac, float denom, double zeta, float scale);
+
 struct LineEpilogue {
     float deltac, denom;
     double zeta, inv_zeta;
@@ -29,6 +49,8 @@ struct LineEpilogue {
     float* bmax;               // per-block maximum of the smoothed field: written by
                                // the box-sum pass, read by the threshold pass to skip
                                // blocks no cell of which can reach the threshold; or NULL
+    DensityThreshold th;       // threshold when `scaled` is NULL (linear input)
+    const ScaledThreshold* scaled;  // nonlinear input: scale and threshold of the radius
 };
 
 struct FusedShape {
@@ -48,8 +70,12 @@ LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double*
 int fused_radius(const FusedShape& sh, const float2* S, float2* T, const float2* tw, const FusedParams& fp,
                  hipStream_t st);
 int fused_lines(const FusedShape& sh, int mode, const float2* U, const float2* tw, const FusedParams& fp,
-                const LineEpilogue& ep, hipStream_t st);
+                const LineEpilogue& ep, hipStream_t st, int stage = CM21_STAGE_FUSED_LINES);
 int fused_mask_clear(unsigned long long* mask, size_t words, hipStream_t st);
-int fused_sum(const double* partial, int n, double* out, hipStream_t st);
+// Fixed-order FP64 sum of the box-sum partials into *out; with `scaled`
+// also the box-mean scale float32(fc_mean / (sum/ncells)) and its threshold.
+int fused_sum(const double* partial, int n, double* out, hipStream_t st, ScaledThreshold* scaled = nullptr,
+              float fc_mean = 0.0f, double ncells = 1.0, float deltac = 0.0f, float denom = 1.0f,
+              double zeta = 1.0);
 
 }  // namespace cm21

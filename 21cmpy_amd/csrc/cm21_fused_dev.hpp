@@ -219,16 +219,9 @@ __device__ inline float fcoll_for_sum(float smoothed, float deltac, float inv_de
     return erfcf((deltac - smoothed) * inv_denom);
 }
 
-// Threshold of the centre-pixel rule in density space.  f = erfcf((deltac -
-// d)/denom) * scale >= 1/zeta holds exactly when d >= d*, with
-// d* = deltac - denom * erfcinv(1/(zeta*scale)), up to the float32 rounding of
-// the left side; cells within `band` of d* take the exact float32 test, the
-// others are decided by comparing d with d*.
-struct DensityThreshold {
-    float dstar;  // decided by d > dstar outside the band
-    float band;   // +inf: every cell takes the exact test
-};
-
+// Threshold of the centre-pixel rule in density space (DensityThreshold):
+// d* = deltac - denom * erfcinv(1/(zeta*scale)), band 1e-4 (1 + |deltac| +
+// denom + |d*|).
 __device__ inline DensityThreshold density_threshold(float deltac, float denom, double zeta, float scale) {
     const double y = 1.0 / (zeta * (double)scale);
     DensityThreshold t;

@@ -29,24 +29,37 @@ __device__ inline uint4 philox4x32_10(uint4 c, uint2 k) {
     return c;
 }
 
-// ((u >> 8) + 0.5) * 2^-24: an exact value in (0, 1).
-__device__ inline double uniform_open01(uint32_t u) { return ((double)(u >> 8) + 0.5) * (1.0 / 16777216.0); }
-
-// FP64 Box-Muller: r = sqrt(-2 ln a), theta = 2 pi b.
-__device__ inline void box_muller(uint32_t u0, uint32_t u1, double& g1, double& g2) {
-    const double a = uniform_open01(u0), b = uniform_open01(u1);
-    const double r = sqrt(-2.0 * log(a));
-    const double th = (2.0 * M_PI) * b;
-    g1 = r * cos(th);
-    g2 = r * sin(th);
+// Box-Muller in float32, r = sqrt(-2 ln a), theta = 2 pi b, with the
+// uniforms a = ((u0 >> 8) + 0.5) 2^-24 and b likewise, handled so that no
+// argument is rounded: a = m 2^-25 with m = 2 (u0 >> 8) + 1 is an exact float
+// when a <= 1/2, and 1 - a is one otherwise (ln a = log1p(-(1 - a))); 2b =
+// m' 2^-24 is exact below 1, and 2b - 1 is exact above (sin and cos change
+// sign), so sincospi sees an exact argument.  Each result is within a few
+// float32 ulp of the FP64 values.
+__device__ inline void box_muller(uint32_t u0, uint32_t u1, float& g1, float& g2) {
+    const uint32_t m0 = 2u * (u0 >> 8) + 1u;
+    const float lna = m0 < (1u << 24) ? logf(ldexpf((float)m0, -25))
+                                      : log1pf(-ldexpf((float)((1u << 25) - m0), -25));
+    const float r = sqrtf(-2.0f * lna);
+    const uint32_t m1 = 2u * (u1 >> 8) + 1u;
+    float s, c;
+    if (m1 < (1u << 24)) {
+        sincospif(ldexpf((float)m1, -24), &s, &c);
+    } else {
+        sincospif(ldexpf((float)(m1 - (1u << 24)), -24), &s, &c);
+        s = -s;
+        c = -c;
+    }
+    g1 = r * c;
+    g2 = r * s;
 }
 
 struct GrfParams {
-    double dk;       // 2 pi / box_len
-    double amp;      // P(k) normalisation
-    double n_index;  // spectral slope
-    double k_cut;    // Gaussian cut-off scale
-    double volume;   // box_len^3
+    float dkf;              // 2 pi / box_len
+    float amp_over_volume;  // P(k) normalisation / box_len^3
+    float n_indexf;         // spectral slope
+    float inv_k_cut;        // 1 / Gaussian cut-off scale
+    float inv_volume;       // 1 / box_len^3
     uint32_t seed, box_id;
     const float* ktab;  // tabulated P(k): increasing K (NULL = power law)
     const float* ptab;
@@ -56,21 +69,20 @@ struct GrfParams {
 constexpr int kMaxTable = 2048;
 
 // P(k) from the table by the rule of initialize.cu:87-90: i is the first
-// index with K[i] >= k and P is interpolated linearly between i-1 and i
-// (in FP64 here); modes outside [K[0], K[n-1]] get 0.
-__device__ inline double table_pk(double k, const float* K, const float* Pt, int n) {
-    if (!(k >= (double)K[0]) || k > (double)K[n - 1]) return 0.0;
+// index with K[i] >= k and P is interpolated linearly between i-1 and i in
+// float32, as init_kernel does; modes outside [K[0], K[n-1]] get 0.
+__device__ inline float table_pk(float k, const float* K, const float* Pt, int n) {
+    if (!(k >= K[0]) || k > K[n - 1]) return 0.0f;
     int lo = 0, hi = n - 1;  // K[lo] <= k <= K[hi]
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if ((double)K[mid] >= k)
+        if (K[mid] >= k)
             hi = mid;
         else
             lo = mid;
     }
-    if ((double)K[lo] >= k) return (double)Pt[lo];  // k == K[0]
-    const double k0 = K[lo], k1 = K[hi], p0 = Pt[lo], p1 = Pt[hi];
-    return p0 + (p1 - p0) * (k - k0) / (k1 - k0);
+    if (K[lo] >= k) return Pt[lo];  // k == K[0]
+    return Pt[lo] + (Pt[hi] - Pt[lo]) * ((k - K[lo]) / (K[hi] - K[lo]));
 }
 
 __global__ void k_grf(float2* __restrict__ spec, int D, size_t Nc, GrfParams gp) {
@@ -103,30 +115,25 @@ __global__ void k_grf(float2* __restrict__ spec, int D, size_t Nc, GrfParams gp)
     const long ra = use_mirror ? ma : a, rb = use_mirror ? mb : b;
     const unsigned long long rep = ((unsigned long long)(ra * D + rb)) * (unsigned long long)H + (unsigned long long)c;
     const uint4 u = philox4x32_10(make_uint4((uint32_t)rep, (uint32_t)(rep >> 32), 0u, 0u), make_uint2(gp.seed, gp.box_id));
-    double g1, g2;
+    float g1, g2;
     box_muller(u.x, u.y, g1, g2);
 
-    const double na = mode_of(a, D), nb = mode_of(b, D), nc = c;
-    const double m2 = na * na + nb * nb + nc * nc;
-    const double k = gp.dk * sqrt(m2);
-    double P;
+    // |n|^2 is an exact integer; k and the amplitude in float32
+    const int na = mode_of(a, D), nb = mode_of(b, D);
+    const float k = gp.dkf * sqrtf((float)(na * na + nb * nb + c * c));
+    float amp;
     if (gp.ntab > 0) {
-        P = table_pk(k, sK, sP, gp.ntab);
+        amp = sqrtf(table_pk(k, sK, sP, gp.ntab) * gp.inv_volume);
     } else {
-        const double x = k / gp.k_cut;
-        P = gp.amp * (pow(k, gp.n_index) * exp(-(x * x)));
+        const float x = k * gp.inv_k_cut;
+        amp = sqrtf(gp.amp_over_volume * (powf(k, gp.n_indexf) * expf(-(x * x))));
     }
-    const double amp = sqrt(P / gp.volume);
-    double re, im;
     if (selfconj) {
-        re = amp * g1;
-        im = 0.0;
+        spec[idx] = make_float2(amp * g1, 0.0f);
     } else {
-        re = amp * g1 / 1.4142135623730951;
-        im = amp * g2 / 1.4142135623730951;
-        if (use_mirror) im = -im;
+        const float h = amp * 0.70710678118654752f;
+        spec[idx] = make_float2(h * g1, use_mirror ? -(h * g2) : h * g2);
     }
-    spec[idx] = make_float2((float)re, (float)im);
 }
 
 __global__ void k_subsample(float* __restrict__ small, const float* __restrict__ large, int D, int sw, float pf) {
@@ -187,8 +194,9 @@ extern "C" int cm21_grf_init(void* spec_out, int D, double box_len, double amp, 
     CM21_REQUIRE(valid_dim(D), "cm21_grf_init: D=%d must be even in [2,2048]", D);
     CM21_REQUIRE(box_len > 0.0 && k_cut > 0.0 && amp >= 0.0, "cm21_grf_init: box_len, k_cut must be > 0, amp >= 0");
     const size_t Nc = (size_t)D * D * (D / 2 + 1);
-    GrfParams gp{2.0 * M_PI / box_len, amp, n_index, k_cut, box_len * box_len * box_len, seed, box_id,
-                 nullptr, nullptr, 0};
+    const double vol = box_len * box_len * box_len;
+    GrfParams gp{(float)(2.0 * M_PI / box_len), (float)(amp / vol), (float)n_index, (float)(1.0 / k_cut),
+                 (float)(1.0 / vol), seed, box_id, nullptr, nullptr, 0};
     k_grf<<<grid_for(Nc), kThreads, 0, (hipStream_t)stream>>>((float2*)spec_out, D, Nc, gp);
     CM21_LAUNCH_CHECK("k_grf");
     return CM21_OK;
@@ -203,8 +211,8 @@ extern "C" int cm21_grf_init_table(void* spec_out, int D, double box_len, const 
     CM21_REQUIRE(n_table >= 2 && n_table <= kMaxTable, "cm21_grf_init_table: n_table=%d not in [2,%d]", n_table,
                  kMaxTable);
     const size_t Nc = (size_t)D * D * (D / 2 + 1);
-    GrfParams gp{2.0 * M_PI / box_len, 0.0, 0.0, 1.0, box_len * box_len * box_len, seed, box_id,
-                 k_table, p_table, n_table};
+    GrfParams gp{(float)(2.0 * M_PI / box_len), 0.0f, 0.0f, 1.0f, (float)(1.0 / (box_len * box_len * box_len)),
+                 seed, box_id, k_table, p_table, n_table};
     k_grf<<<grid_for(Nc), kThreads, 0, (hipStream_t)stream>>>((float2*)spec_out, D, Nc, gp);
     CM21_LAUNCH_CHECK("k_grf");
     return CM21_OK;

@@ -89,6 +89,12 @@ __global__ void k_move_ngp(float* __restrict__ acc, const float* __restrict__ de
 // kHalo cells (plus one for the CIC neighbour) and flushes each non-zero tile
 // cell with one global atomic; deposits outside the tile go straight to
 // global memory.  scheme 0 = NGP, 1 = CIC.
+//
+// CIC: along a thread's walk in i the particles advance by 1/PIXEL_FACTOR
+// HII cells in x and share (y, z) cells while their displacement cells do, so
+// the thread keeps the deposits of the two x cells {base, base+1} times the
+// four (y, z) cells in registers and writes a cell to the tile only when the
+// walk leaves it: about 40 LDS atomics per 16 particles instead of 128.
 constexpr int kPatch = 16;
 constexpr int kHalo = 4;
 
@@ -96,6 +102,35 @@ __device__ inline int mod_sw(int x, int sw) {
     x %= sw;
     return x < 0 ? x + sw : x;
 }
+
+// One deposit into the block's tile, or into global memory when the cell
+// lies outside the tile.  x, y, z are unwrapped HII cells.
+__device__ __forceinline__ void deposit(float* tile, float* acc, int x, int y, int z, float w, int ox, int oy,
+                                        int oz, int T, int sw, bool use_tile) {
+    if (w == 0.0f) return;
+    const int lx = mod_sw(x - ox, sw), ly = mod_sw(y - oy, sw), lz = mod_sw(z - oz, sw);
+    if (use_tile && lx < T && ly < T && lz < T)
+        atomicAdd(&tile[(lx * T + ly) * T + lz], w);
+    else
+        atomicAdd(&acc[((size_t)mod_sw(x, sw) * sw + mod_sw(y, sw)) * sw + mod_sw(z, sw)], w);
+}
+
+// The register accumulator of one thread's CIC walk: cells (base + a, y0 + b,
+// z0 + c) for a, b, c in {0, 1}.
+struct CicRun {
+    int base, y0, z0;
+    bool live;
+    float v[2][4];  // [a][2b + c]
+
+    __device__ __forceinline__ void flush_x(int a, float* tile, float* acc, int ox, int oy, int oz, int T, int sw,
+                                            bool use_tile) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            deposit(tile, acc, base + a, y0 + (q >> 1), z0 + (q & 1), v[a][q], ox, oy, oz, T, sw, use_tile);
+            v[a][q] = 0.0f;
+        }
+    }
+};
 
 template <int SCHEME>
 __global__ void k_move_tile(float* __restrict__ acc, const float* __restrict__ delta, const float* __restrict__ vx,
@@ -116,6 +151,11 @@ __global__ void k_move_tile(float* __restrict__ acc, const float* __restrict__ d
         for (int q = t; q < T3; q += blockDim.x) tile[q] = 0.0f;
         __syncthreads();
     }
+    CicRun run;
+    run.live = false;
+    run.base = run.y0 = run.z0 = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) run.v[0][q] = run.v[1][q] = 0.0f;
     for (int ii = 0; ii < kPatch; ++ii) {
         const int i = bi * kPatch + ii;
         const size_t p = ((size_t)i * DIM + j) * DIM + k;
@@ -135,27 +175,39 @@ __global__ void k_move_tile(float* __restrict__ acc, const float* __restrict__ d
             const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
             const float tx = __fsub_rn(cx, fx), ty = __fsub_rn(cy, fy), tz = __fsub_rn(cz, fz);
             const int x0 = (int)fx, y0 = (int)fy, z0 = (int)fz;
-#pragma unroll
-            for (int dx = 0; dx < 2; ++dx) {
-                const float wx = dx ? tx : __fsub_rn(1.0f, tx);
-                const int gx = mod_sw(x0 + dx, sw), lx = mod_sw(x0 + dx - ox, sw);
-#pragma unroll
-                for (int dy = 0; dy < 2; ++dy) {
-                    const float wy = dy ? ty : __fsub_rn(1.0f, ty);
-                    const int gy = mod_sw(y0 + dy, sw), ly = mod_sw(y0 + dy - oy, sw);
-#pragma unroll
-                    for (int dz = 0; dz < 2; ++dz) {
-                        const float wz = dz ? tz : __fsub_rn(1.0f, tz);
-                        const int gz = mod_sw(z0 + dz, sw), lz = mod_sw(z0 + dz - oz, sw);
-                        const float w = __fmul_rn(__fmul_rn(__fmul_rn(m, wx), wy), wz);
-                        if (use_tile && lx < T && ly < T && lz < T)
-                            atomicAdd(&tile[(lx * T + ly) * T + lz], w);
-                        else
-                            atomicAdd(&acc[((size_t)gx * sw + gy) * sw + gz], w);
-                    }
-                }
+            if (run.live && (y0 != run.y0 || z0 != run.z0 || x0 < run.base || x0 > run.base + 1)) {
+                run.flush_x(0, tile, acc, ox, oy, oz, T, sw, use_tile);
+                run.flush_x(1, tile, acc, ox, oy, oz, T, sw, use_tile);
+                run.live = false;
             }
+            if (!run.live) {
+                run.base = x0;
+                run.y0 = y0;
+                run.z0 = z0;
+                run.live = true;
+            } else if (x0 == run.base + 1) {
+                // the walk leaves cell `base`: write it, slide the window
+                run.flush_x(0, tile, acc, ox, oy, oz, T, sw, use_tile);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    run.v[0][q] = run.v[1][q];
+                    run.v[1][q] = 0.0f;
+                }
+                run.base = x0;
+            }
+            const float wx[2] = {__fsub_rn(1.0f, tx), tx};
+            const float wy[2] = {__fsub_rn(1.0f, ty), ty};
+            const float wz[2] = {__fsub_rn(1.0f, tz), tz};
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    run.v[dx][q] += __fmul_rn(__fmul_rn(__fmul_rn(m, wx[dx]), wy[q >> 1]), wz[q & 1]);
         }
+    }
+    if (SCHEME == 1 && run.live) {
+        run.flush_x(0, tile, acc, ox, oy, oz, T, sw, use_tile);
+        run.flush_x(1, tile, acc, ox, oy, oz, T, sw, use_tile);
     }
     if (!use_tile) return;
     __syncthreads();

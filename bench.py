@@ -46,6 +46,7 @@ def alg_bytes(D, norm=1):
         "fused_columns": 16 * Nc,
         "fused_rows": 0,
         "fused_lines": 8 * Nc + 4 * N + (4 * N if norm else 12 * N),
+        "fused_lines_threshold": 12 * N,
     }
 
 
@@ -55,35 +56,86 @@ def b_alg(D, n_r, norm):
     return 4 * N + 8 * Nc + n_r * (24 * Nc + (16 + 4 * norm) * N)
 
 
-def cpu_baseline(delta, L, radii, denoms, zeta, deltac, fc_mean, budget_s=15.0):
-    """NumPy FP32 restatement (the oracle) on the same box: time the forward
-    transform plus as many radii as fit the budget; extrapolate to one box."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_loop(delta, L, radii, denoms, zeta, deltac, fc_mean, LE, workers, budget_s):
+    """The oracle's FP32 radius loop (oracle/bubbles.py, the same windows,
+    f_coll and threshold) with scipy.fft transforms on `workers` threads, on
+    the host cores: the forward transform plus as many radii as fit the
+    budget (all of them when budget_s is None).  Returns (seconds per box
+    extrapolated to every radius, radii timed, ionized box)."""
+    import scipy.fft as sfft
     from oracle import bubbles
     D = delta.shape[0]
     t0 = time.perf_counter()
-    d = (delta * np.float32(deltac / 1.686)).astype(np.float32)
-    spec = np.fft.rfftn(d.astype(np.float64)).astype(np.complex64)
+    d = delta if LE else (delta * np.float32(deltac / 1.686)).astype(np.float32)
+    spec = sfft.rfftn(d.astype(np.float64), workers=workers).astype(np.complex64)
     km = bubbles.kmag_half_f32(D, bubbles.delta_k_f32(L))
     t_fwd = time.perf_counter() - t0
     done, t_rad = 0, 0.0
     ion = np.zeros(delta.shape, np.float32)
-    for R, den in zip(radii, denoms):
+    for r, (R, den) in enumerate(zip(radii, denoms)):
         t1 = time.perf_counter()
         w0, cut, w2 = bubbles.window_f32(km, R, 1, True)
         filt = bubbles.apply_window_f32(spec, w0, cut, w2)
-        sm = np.fft.irfftn(filt.astype(np.complex128), s=(D, D, D), axes=(0, 1, 2)).astype(np.float32)
+        sm = sfft.irfftn(filt.astype(np.complex128), s=(D, D, D), axes=(0, 1, 2), workers=workers).astype(np.float32)
         fc = bubbles.fcoll_f32(sm, deltac, den)
-        fc = (fc * np.float32(float(fc_mean) / (fc.astype(np.float64).sum() / D ** 3))).astype(np.float32)
-        ion[fc.astype(np.float64) >= 1.0 / zeta] = 1.0
+        if not LE:
+            fc = (fc * bubbles.box_mean_scale_f32(fc, fc_mean)).astype(np.float32)
+        hit = fc.astype(np.float64) >= 1.0 / zeta
+        if r == len(radii) - 1:
+            todo = ion != 1.0
+            ion[todo & hit] = 1.0
+            part = todo & ~hit
+            ion[part] = (fc[part].astype(np.float64) * zeta).astype(np.float32)
+        else:
+            ion[hit] = 1.0
         t_rad += time.perf_counter() - t1
         done += 1
-        if t_fwd + t_rad > budget_s:
+        if budget_s is not None and t_fwd + t_rad > budget_s:
             break
-    per_box = t_fwd + t_rad / done * len(radii)
-    return {"value": 1.0 / per_box, "unit": "boxes/s", "cores": 1, "kind": "port",
-            "sample": f"NumPy FP32 oracle (oracle/bubbles.py, pocketfft single thread) on the same {D}^3 box: "
-                      f"forward transform + {done} of {len(radii)} radii timed ({t_fwd + t_rad:.1f} s), "
-                      f"extrapolated to one box"}
+    return t_fwd + t_rad / done * len(radii), done, ion
+
+
+def cpu_baseline(delta, L, radii, denoms, zeta, deltac, fc_mean, budget_s=15.0, gpu_config1=None):
+    """The same algorithm on the host cores (kind "port": the oracle's FP32
+    restatement), timed on a bounded sample of the benchmark box with scipy.fft
+    on every available core (the reported value) and on one thread, plus
+    configuration 1 (64^3, 37 radii, linear input) timed in full."""
+    from oracle import grf, sigma
+    ncores = len(os.sched_getaffinity(0))
+    D = delta.shape[0]
+    per_box_mt, done_mt, _ = cpu_loop(delta, L, radii, denoms, zeta, deltac, fc_mean, False, ncores, budget_s)
+    per_box_1, done_1, _ = cpu_loop(delta, L, radii, denoms, zeta, deltac, fc_mean, False, 1, budget_s)
+    # configuration 1 in full: D=64, L=100, seed 0, sigma_min^2 = 4, zeta = 30, deltac = 1.686, linear input
+    d1 = grf.c2r(grf.grf_half_spectrum(64, 100.0, seed=0), 64).astype(np.float32)
+    r1, _ = sigma.radius_schedule(100.0, 64)
+    den1 = sigma.denominators(r1, 4.0, lambda R: sigma.sig0_discrete(R, 64, 100.0))
+    t0 = time.perf_counter()
+    _, _, ion1 = cpu_loop(d1, 100.0, r1, den1, 30.0, 1.686, 0.0, True, 1, None)
+    t_c1 = time.perf_counter() - t0
+    out = {"value": 1.0 / per_box_mt, "unit": "boxes/s", "cores": ncores, "kind": "port",
+           "cpu_model": cpu_model(), "threads": {"scipy.fft workers": ncores, "numpy elementwise": 1},
+           "sample": f"oracle FP32 radius loop (oracle/bubbles.py windows, f_coll, threshold) with scipy.fft on "
+                     f"{ncores} cores on the same {D}^3 benchmark box: forward transform + {done_mt} of "
+                     f"{len(radii)} radii timed (budget {budget_s:.0f} s), extrapolated to one box",
+           "single_thread": {"value": 1.0 / per_box_1, "unit": "boxes/s", "cores": 1,
+                             "sample": f"{done_1} of {len(radii)} radii, scipy.fft workers=1, extrapolated"},
+           "config1_full": {"value": 1.0 / t_c1, "unit": "boxes/s", "cores": 1, "seconds": t_c1,
+                            "n_radii": len(r1), "x_HI": float(1.0 - ion1.mean()),
+                            "workload": "configuration 1: 64^3 GRF seed 0, BOX_LEN 100, 37 radii, sigma_min^2 4, "
+                                        "zeta 30, linear input, every radius timed, one thread"}}
+    if gpu_config1:
+        out["config1_full"]["gpu_boxes_s"] = gpu_config1
+    return out
 
 
 def loop_only(args, lib, torch):
@@ -103,16 +155,35 @@ def loop_only(args, lib, torch):
     print(json.dumps({"mode": "loop", "D": D, "steps": args.steps}))
 
 
-def batched_bench(args, torch):
+def partly_ionized_specs(specs, seed=1):
+    """The manifest's boxes with sigma_min^2 = (deltac/u)^2/2, u uniform in
+    [1.3, 1.7] from NumPy PCG64(seed): zeta erfc(deltac/sqrt(2 sigma_min^2))
+    straddles 1, so every box is partly ionized."""
+    bm = importlib.import_module("21cmpy_amd.Programs.find_HII_bubbles_batched")
+    rng = np.random.Generator(np.random.PCG64(int(seed)))
+    out = []
+    for s in specs:
+        s = dict(s)
+        dc = 1.686 / bm.linear_growth(s["z"])
+        s["sigma_min2"] = (dc / rng.uniform(1.3, 1.7)) ** 2 / 2.0
+        out.append(s)
+    return out
+
+
+def batched_bench(args, torch, partial=False):
     """The 64-box batch (mixed HII_DIM 96/128/160, per-box zeta, z, sigma_min^2
     from the committed manifest; linear z=0 GRF boxes, cell 1.5625 Mpc, 37
-    radii, HII_FILTER 1) through cm21_find_bubbles_batched as one graph.
-    Returns the result object for the JSON line."""
+    radii, HII_FILTER 1) through cm21_find_bubbles_batched as one graph; with
+    partial, the same boxes with sigma_min^2 chosen so that each is partly
+    ionized (partly_ionized_specs).  Returns the result object for the JSON
+    line."""
     bm = importlib.import_module("21cmpy_amd.Programs.find_HII_bubbles_batched")
     g = importlib.import_module("21cmpy_amd.gpu")
     sig = importlib.import_module("21cmpy_amd.cosmo_files.sigmas")
     lib = importlib.import_module("21cmpy_amd._lib")
     specs = bm.load_manifest()["boxes"]
+    if partial:
+        specs = partly_ionized_specs(specs)
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         bb = bm.BatchedBubbles(specs, fil=1, LE=True, graph=not args.no_graph, stream=stream)
@@ -122,7 +193,7 @@ def batched_bench(args, torch):
             if key not in amps:
                 amps[key] = sig.PowerLawSpectrum(*key).amp
             bb.set_delta(i, g.c2r(g.grf_init(s["D"], s["box_len"], amps[key], s["seed"])))
-        tuned = None if args.no_tune else bb.autotune(reps=3, save_path=args.save_tiles)
+        tuned = None if (args.no_tune or partial) else bb.autotune(reps=3, save_path=args.save_tiles)
     torch.cuda.synchronize()
     for _ in range(max(args.warmup, 1)):
         bb.run()
@@ -145,18 +216,40 @@ def batched_bench(args, torch):
     fb = importlib.import_module("21cmpy_amd.Programs.find_HII_bubbles")
     balg = sum(b_alg(s["D"], len(fb.radius_schedule(s["box_len"], s["D"])[0]), 0) for s in specs)
     xhi = bb.x_HI()
+    traffic = read_traffic("batched_find_bubbles_mix64_traffic.txt", "batch_bytes")
     return {
         "value": value, "unit": "boxes/s", "boxes_per_batch": nb, "steps": steps,
         "ms_per_batch": elapsed * 1e3 / steps,
         "workload": "64 linear GRF boxes, HII_DIM in {96,128,160} (21/25/18), BOX_LEN = 1.5625 Mpc x HII_DIM, "
                     "37 radii, per-box zeta in [10,60], z in {7..10} (deltac = 1.686/D(z), D(z)=0.12*9/(1+z)), "
-                    "sigma_min^2 in [2,6], HII_FILTER 1; one graph per batch",
-        "model_roofline": {"B_alg_per_batch": balg, "frac": balg * steps / elapsed / (HBM_PEAK_GBS * 1e9)},
+                    + ("sigma_min^2 = (deltac/u)^2/2 with u in [1.3,1.7] (partly ionized)" if partial else
+                       "sigma_min^2 in [2,6]") + ", HII_FILTER 1; one graph per batch",
+        # the unit model credits every box with the full per-radius passes;
+        # the batched loop prunes them and keeps the mask on chip, so this
+        # ratio is a model-bytes rate, not a bandwidth
+        "model_bytes_rate": {"B_alg_per_batch": balg, "model_bytes_per_s_over_hbm_peak":
+                             balg * steps / elapsed / (HBM_PEAK_GBS * 1e9)},
+        "hbm_traffic": ({"bytes_per_batch": traffic, "gbs": traffic * steps / elapsed / 1e9,
+                         "frac": traffic * steps / elapsed / (HBM_PEAK_GBS * 1e9),
+                         "source": "profiles/batched_find_bubbles_mix64_traffic.txt (literal parameters)"}
+                        if traffic and not partial else None),
         "stages_one_batch_eager": stages,
         "tiles": ({k: v["chosen"] for k, v in tuned.items()} if tuned else
                   {k: {"columns": c, "lines": l} for k, (c, l) in bb.tiles().items()}),
         "x_HI_mean": float(np.mean(xhi)), "x_HI_min": float(np.min(xhi)),
     }
+
+
+def read_traffic(name, key):
+    """A calibrated HBM byte count from a committed rocprofv3 counter summary
+    under profiles/ (None when absent)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(key + ":"):
+            return float(line.split(":", 1)[1].split()[0])
+    return None
 
 
 def unet_flops(D):
@@ -171,9 +264,10 @@ def unet_flops(D):
     return mfma, other
 
 
-def plain_rate(torch, delta, L, deltac, smin, zeta, steps, linear=True):
-    """Boxes/s of the plain single-box loop (fused path, graph replay) on one
-    resident box, radii and denominators computed beforehand."""
+def plain_rate(torch, delta, L, deltac, smin, zeta, steps, linear=True, fused=True, graph=True, warmup=3):
+    """Boxes/s of the plain single-box loop (graph replay unless graph=False)
+    on one resident box, radii and denominators computed beforehand; returns
+    (boxes/s, ionized box)."""
     lib = importlib.import_module("21cmpy_amd._lib")
     fb = importlib.import_module("21cmpy_amd.Programs.find_HII_bubbles")
     sig = importlib.import_module("21cmpy_amd.cosmo_files.sigmas")
@@ -182,7 +276,7 @@ def plain_rate(torch, delta, L, deltac, smin, zeta, steps, linear=True):
     prm = lib.BubbleParams(box_len=L, deltac=float(np.float32(deltac)), zeta=zeta,
                            growth_inv=float(np.float32(deltac / 1.686)), fc_mean=float(fc_mean),
                            linear_input=1 if linear else 0, filter_type=1, ref_fallthrough=1, update=0,
-                           sphere_periodic=0, sphere_r_scale=1.0, fused=1, graph=1)
+                           sphere_periodic=0, sphere_r_scale=1.0, fused=1 if fused else 0, graph=1 if graph else 0)
     plan = lib.plan(D, 1)
     nbytes = lib.query("cm21_find_bubbles_workspace_bytes", D, len(radii), 0)
     work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
@@ -190,6 +284,7 @@ def plain_rate(torch, delta, L, deltac, smin, zeta, steps, linear=True):
     r_arr = np.ascontiguousarray(radii, np.float32)
     d_arr = np.ascontiguousarray(denoms, np.float32)
     stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
 
     def step():
         lib.call("cm21_find_bubbles", plan.handle, lib.ptr(delta), lib.ptr(ion), r_arr.ctypes.data_as(lib.c_float_p),
@@ -197,14 +292,66 @@ def plain_rate(torch, delta, L, deltac, smin, zeta, steps, linear=True):
                  lib.stream_handle(stream))
 
     torch.cuda.synchronize()
-    for _ in range(3):
+    for _ in range(warmup):
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     torch.cuda.synchronize()
-    return steps / (time.perf_counter() - t0)
+    rate = steps / (time.perf_counter() - t0)
+    return rate, ion
+
+
+def flag_compare(a, b):
+    """Fraction of cells whose ionized flag differs and |x_HI(a) - x_HI(b)|."""
+    fa, fb_ = (a == 1), (b == 1)
+    flips = float((fa != fb_).double().mean())
+    dx = abs(float(a.double().mean()) - float(b.double().mean()))
+    return flips, dx
+
+
+def extra_workloads(torch, upd, steps):
+    """Throughput beside the headline: the headline box with a barrier that
+    ionizes part of it (the threshold band and the mask writes do real work),
+    checked fused against the rocFFT path; 512^3 boxes (literal and partly
+    ionized parameters); configuration 1 (64^3) on the GPU."""
+    g = importlib.import_module("21cmpy_amd.gpu")
+    sig = importlib.import_module("21cmpy_amd.cosmo_files.sigmas")
+    out = {}
+    # headline box, nonlinear input, barrier 1.686/0.12, sigma_min^2 = 40: partly ionized
+    L, growth, zeta, smin = 300.0, 0.12, 30.0, 40.0
+    rate, ion = plain_rate(torch, upd, L, 1.686 / growth, smin, zeta, steps, linear=False)
+    _, ion_g = plain_rate(torch, upd, L, 1.686 / growth, smin, zeta, 1, linear=False, fused=False, graph=False,
+                          warmup=0)
+    flips, dx = flag_compare(ion, ion_g)
+    out["config2_partly_ionized"] = {
+        "value": rate, "unit": "boxes/s", "x_HI": 1.0 - float(ion.double().mean()),
+        "flag_flips_vs_rocfft_path": flips, "abs_dx_HI_vs_rocfft_path": dx,
+        "workload": "the headline 256^3 nonlinear box, zeta 30, barrier 1.686/0.12, sigma_min^2 = 40 "
+                    "(partly ionized), 40 radii, graph replay"}
+    del ion, ion_g
+    # 512^3, BOX_LEN 500, 41 radii, linear input, zeta 25 (configuration 4's box)
+    D, L = 512, 500.0
+    sp = sig.PowerLawSpectrum(D, L)
+    box = g.c2r(g.grf_init(D, L, sp.amp, 0))
+    for name, smin in (("plain_512_literal", 4.0), ("plain_512_partly_ionized", 0.6)):
+        rate, ion = plain_rate(torch, box, L, 1.686, smin, 25.0, max(steps // 4, 3), linear=True, warmup=1)
+        out[name] = {"value": rate, "unit": "boxes/s", "x_HI": 1.0 - float(ion.double().mean()),
+                     "workload": f"512^3 linear GRF box (seed 0, BOX_LEN 500), 41 radii, zeta 25, "
+                                 f"sigma_min^2 = {smin}, deltac 1.686, graph replay"}
+        del ion
+    del box
+    torch.cuda.empty_cache()
+    # configuration 1 on the GPU
+    D, L = 64, 100.0
+    sp = sig.PowerLawSpectrum(D, L)
+    box = g.c2r(g.grf_init(D, L, sp.amp, 0))
+    rate, ion = plain_rate(torch, box, L, 1.686, 4.0, 30.0, steps * 5, linear=True)
+    out["config1"] = {"value": rate, "unit": "boxes/s", "x_HI": 1.0 - float(ion.double().mean()),
+                      "workload": "configuration 1: 64^3 GRF seed 0, BOX_LEN 100, 37 radii, sigma_min^2 4, "
+                                  "zeta 30, linear input, graph replay"}
+    return out
 
 
 def surrogate_bench(args, torch, plain_boxes_s=None):
@@ -253,8 +400,8 @@ def surrogate_bench(args, torch, plain_boxes_s=None):
     # plain method on the same box size, one box per call through the C ABI
     # (graph replay), host set-up outside the timed region
     s0 = specs[0]
-    plain128 = plain_rate(torch, boxes.delta[0].contiguous(), L, s0["deltac"], s0["sigma_min2"], s0["zeta"],
-                          steps)
+    plain128, _ = plain_rate(torch, boxes.delta[0].contiguous(), L, s0["deltac"], s0["sigma_min2"], s0["zeta"],
+                             steps)
     mf, other = unet_flops(D)
     achieved = B * (mf + other) / (fwd_ms * 1e-3) / 1e12
     out = {
@@ -286,6 +433,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="batched: use the stored tile choice, no autotuning")
     ap.add_argument("--save-tiles", default=None, help="batched: merge the autotuned tiles into this JSON file")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the U-Net surrogate measurement")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the partly ionized, 512^3 and configuration-1 throughput measurements")
     ap.add_argument("--mode", choices=["bench", "loop", "copy", "batched", "surrogate"], default="bench",
                     help="loop: only the radius loop on a GRF box (for counter runs); copy: a 512 MiB device "
                          "copy (to calibrate FETCH_SIZE/WRITE_SIZE)")
@@ -409,12 +558,9 @@ def main():
         v["gbs_alg"] = (ab[k] / (v["ms_per_launch"] * 1e-3) / 1e9) if k in ab else None
     dom_ms = prof[dom][0] / prof[dom][1]
     achieved = ab[dom] / (dom_ms * 1e-3) / 1e9
-    traffic = None  # calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, when a summary exists
-    tfile = os.path.join(ROOT, "profiles", f"{dom}_D{D}_traffic.txt")
-    if os.path.exists(tfile) and not args.generic:
-        for line in open(tfile):
-            if line.startswith("bytes_per_launch:"):
-                traffic = float(line.split(":", 1)[1])
+    # calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of the same
+    # kernel and workload, from the committed counter summary
+    traffic = None if args.generic else read_traffic(f"{dom}_D{D}_traffic.txt", "bytes_per_launch")
 
     # delta_T and P21(k) in 40 log bins k_fund..k_nyq on the result (once)
     def post():
@@ -445,6 +591,8 @@ def main():
                    "path": ("plain-generic" if args.generic else "plain-fused") + ("" if args.no_graph else "+graph")},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_gbs": (traffic / (dom_ms * 1e-3) / 1e9) if traffic else None,
+                     "traffic_frac": (traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
                      "alg_bytes_per_launch": ab[dom], "ms_per_launch": dom_ms},
         "model_roofline": {"B_alg_per_box": b_alg(D, len(radii), 1),
                            "frac": b_alg(D, len(radii), 1) * boxes_s / (HBM_PEAK_GBS * 1e9)},
@@ -452,18 +600,26 @@ def main():
         "pipeline_ms": {"initialize_512_to_256": t_init, "perturb_cic": t_pert, "delta_T_and_P21": t_post},
         "x_HI": x_HI,
     }
-    if not args.no_batched:
+    if not args.no_extra:
         del work, ion
+        torch.cuda.empty_cache()
+        out["extra"] = extra_workloads(torch, upd, args.steps)
+        out["extra"]["config2_partly_ionized"]["vs_literal"] = out["extra"]["config2_partly_ionized"]["value"] / boxes_s
+    if not args.no_batched:
         torch.cuda.empty_cache()
         out["batched"] = batched_bench(args, torch)
         out["batched"]["vs_plain"] = out["batched"]["value"] / boxes_s
+        out["batched_partly_ionized"] = batched_bench(args, torch, partial=True)
+        if "extra" in out:
+            out["batched_partly_ionized"]["vs_plain_partly_ionized"] = (
+                out["batched_partly_ionized"]["value"] / out["extra"]["config2_partly_ionized"]["value"])
     if not args.no_surrogate:
         torch.cuda.empty_cache()
         out["surrogate"] = surrogate_bench(args, torch, boxes_s)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(upd.cpu().numpy(), L, radii, denoms, zeta, deltac, fc_mean,
-                                           args.cpu_budget)
-        out["cpu_baseline"]["cores_available"] = len(os.sched_getaffinity(0))
+                                           args.cpu_budget,
+                                           out.get("extra", {}).get("config1", {}).get("value"))
     print(json.dumps(out))
 
 

@@ -219,8 +219,10 @@ int cm21_power_spectrum(cm21_plan* plan, const float* field,
 #define CM21_STAGE_SPHERE 5
 #define CM21_STAGE_FUSED_COLUMNS 6
 #define CM21_STAGE_FUSED_ROWS 7
-#define CM21_STAGE_FUSED_LINES 8
-#define CM21_NSTAGES 9
+#define CM21_STAGE_FUSED_LINES 8           /* the first lines pass of a radius (box sum for
+                                              nonlinear input, else the threshold/paint) */
+#define CM21_STAGE_FUSED_LINES_THRESHOLD 9 /* the threshold/paint lines pass after a box sum */
+#define CM21_NSTAGES 10
 int cm21_profile_enable(int stage_mask);
 int cm21_profile_collect(double* ms, int64_t* count, int nstages);
 

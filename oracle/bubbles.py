@@ -234,12 +234,15 @@ def find_bubbles_f64(delta, box_len, radii, denoms, zeta, deltac, LE=True,
                      growth_inv=1.0, fc_mean=None, fil=1, ref_fallthrough=True, fft_workers=None):
     """FP64 version of the centre-pixel loop; also returns, per cell, the
     minimum over radii of |zeta*f_coll - 1| (the threshold margin).
-    fft_workers: run the transforms with scipy.fft on that many threads
-    (same results up to FP64 rounding; for large boxes)."""
+    fft_workers: run the transforms with scipy.fft on that many threads and
+    the per-cell work on as many threads over slabs of axis 0 (same results up
+    to FP64 rounding; for large boxes)."""
+    import scipy.fft as sfft
+    from concurrent.futures import ThreadPoolExecutor
+    workers = int(fft_workers) if fft_workers else 1
     if fft_workers:
-        import scipy.fft as sfft
-        rfftn = lambda a: sfft.rfftn(a, workers=fft_workers)  # noqa: E731
-        irfftn = lambda a, s: sfft.irfftn(a, s=s, axes=(0, 1, 2), workers=fft_workers)  # noqa: E731
+        rfftn = lambda a: sfft.rfftn(a, workers=workers)  # noqa: E731
+        irfftn = lambda a, s: sfft.irfftn(a, s=s, axes=(0, 1, 2), workers=workers)  # noqa: E731
     else:
         rfftn = np.fft.rfftn
         irfftn = lambda a, s: np.fft.irfftn(a, s=s, axes=(0, 1, 2))  # noqa: E731
@@ -254,21 +257,53 @@ def find_bubbles_f64(delta, box_len, radii, denoms, zeta, deltac, LE=True,
     kmag = dk * np.sqrt(n0[:, None, None] ** 2 + n0[None, :, None] ** 2 + n2[None, None, :] ** 2)
     ionized = np.zeros((D, D, D))
     margin = np.full((D, D, D), np.inf)
+    fc = np.empty((D, D, D))
     nR = len(radii)
-    for r, (R, denom) in enumerate(zip(radii, denoms)):
-        w = window_f64(kmag, float(R), fil, ref_fallthrough)
-        sm = irfftn(spec * w, (D, D, D))
-        fc = _erfc((float(deltac) - sm) / float(denom))
-        if not LE:
+    step = -(-D // workers)
+    slabs = [slice(i, min(i + step, D)) for i in range(0, D, step)]
+    ex = ThreadPoolExecutor(workers) if workers > 1 else None
+
+    def each(fn):
+        return list(ex.map(fn, slabs)) if ex else [fn(sl) for sl in slabs]
+
+    try:
+        for r, (R, denom) in enumerate(zip(radii, denoms)):
+            filt = np.empty_like(spec)
+
+            def window(sl, R=R):
+                filt[sl] = spec[sl] * window_f64(kmag[sl], float(R), fil, ref_fallthrough)
+
+            each(window)
+            sm = irfftn(filt, (D, D, D))
+            del filt
+
+            def fcoll(sl, sm=sm, denom=denom):
+                fc[sl] = _erfc((float(deltac) - sm[sl]) / float(denom))
+                return float(np.sum(fc[sl]))
+
+            total = sum(each(fcoll))
+            scale = None
+            if not LE:
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    scale = np.float64(fc_mean) / (np.float64(total) / fc.size)
+            final = r == nR - 1
+
+            def update(sl, scale=scale, final=final):
+                f = fc[sl] if scale is None else fc[sl] * scale
+                np.minimum(margin[sl], np.abs(zeta * f - 1.0), out=margin[sl])
+                hit = f >= 1.0 / zeta
+                ion = ionized[sl]
+                if final:
+                    todo = ion != 1.0
+                    ion[todo & hit] = 1.0
+                    part = todo & ~hit
+                    ion[part] = f[part] * zeta
+                else:
+                    ion[hit] = 1.0
+
             with np.errstate(invalid="ignore"):
-                fc = fc * box_mean_scale_f64(fc, fc_mean)
-        margin = np.minimum(margin, np.abs(zeta * fc - 1.0))
-        hit = fc >= 1.0 / zeta
-        if r == nR - 1:
-            todo = ionized != 1.0
-            ionized[todo & hit] = 1.0
-            part = todo & ~hit
-            ionized[part] = fc[part] * zeta
-        else:
-            ionized[hit] = 1.0
+                each(update)
+    finally:
+        if ex:
+            ex.shutdown()
     return ionized, margin

@@ -19,7 +19,7 @@ from oracle import bubbles, deltat, grf, perturb, sigma
 
 pytestmark = pytest.mark.gpu
 
-DIM, HII, L, Z = 64, 32, 64.0, 8.0
+DIM, HII, L, Z = 64, 32, 64.0, 10.0
 
 
 def test_stage_chain_through_files(gpu, tmp_path):
