@@ -114,21 +114,46 @@ __global__ void k_pk_bin(const float2* __restrict__ F, int D, size_t Nc, const i
     }
 }
 
-__global__ void k_pk_final(const double* __restrict__ part_ps, const double* __restrict__ part_k,
-                           const unsigned long long* __restrict__ part_n, int nblocks, int nbins,
-                           double* __restrict__ ps, double* __restrict__ ks, unsigned long long* __restrict__ n) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nbins) return;
+// One block per bin: each thread adds a strided subset of the per-block
+// partials in a fixed order, then a fixed tree over lanes and waves
+// (deterministic).
+__global__ __launch_bounds__(kThreads) void k_pk_final(const double* __restrict__ part_ps,
+                                                       const double* __restrict__ part_k,
+                                                       const unsigned long long* __restrict__ part_n, int nblocks,
+                                                       int nbins, double* __restrict__ ps, double* __restrict__ ks,
+                                                       unsigned long long* __restrict__ n) {
+    __shared__ double wa[kWaves], wb[kWaves];
+    __shared__ unsigned long long wc[kWaves];
+    const int q = blockIdx.x;
     double a = 0.0, b = 0.0;
     unsigned long long c = 0;
-    for (int i = 0; i < nblocks; ++i) {
+    for (int i = threadIdx.x; i < nblocks; i += kThreads) {
         a += part_ps[(size_t)i * nbins + q];
         b += part_k[(size_t)i * nbins + q];
         c += part_n[(size_t)i * nbins + q];
     }
-    ps[q] = a;
-    ks[q] = b;
-    n[q] = c;
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum_u64(c);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        wa[w] = a;
+        wb[w] = b;
+        wc[w] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0.0, tb = 0.0;
+        unsigned long long tc = 0;
+        for (int v = 0; v < kWaves; ++v) {
+            ta += wa[v];
+            tb += wb[v];
+            tc += wc[v];
+        }
+        ps[q] = ta;
+        ks[q] = tb;
+        n[q] = tc;
+    }
 }
 
 constexpr int kPkBlocks = 512;
@@ -211,8 +236,8 @@ extern "C" int cm21_power_spectrum(cm21_plan* plan, const float* field, const do
     k_pk_bin<<<kPkBlocks, kThreads, lds, st>>>(w.spec, D, Nc, bin_lut, nbins, 2.0 * M_PI / box_len,
                                                1.0 / (2.0 * M_PI * M_PI * V), w.part_ps, w.part_k, w.part_n);
     CM21_LAUNCH_CHECK("k_pk_bin");
-    k_pk_final<<<grid_for(nbins, 64), 64, 0, st>>>(w.part_ps, w.part_k, w.part_n, kPkBlocks, nbins, ps_sum, k_sum,
-                                                   (unsigned long long*)counts);
+    k_pk_final<<<nbins, kThreads, 0, st>>>(w.part_ps, w.part_k, w.part_n, kPkBlocks, nbins, ps_sum, k_sum,
+                                           (unsigned long long*)counts);
     CM21_LAUNCH_CHECK("k_pk_final");
     return CM21_OK;
 }

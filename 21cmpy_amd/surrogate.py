@@ -175,10 +175,38 @@ class UNet3D:
         np.savez(path, **{k: v.detach().cpu().numpy() for k, v in self.p.items()},
                  __meta__=np.array(json.dumps(meta)))
 
+    def save_checkpoint(self, path, rng=None, **meta):
+        """A training checkpoint: the fp32 master weights, the AdamW moments m
+        and v, the step count (bias correction and schedule) and, given the
+        training generator, its PCG64 state, so that a run resumed with
+        load_checkpoint continues the same sequence of batches and updates."""
+        state = {"step_count": int(self.step_count)}
+        if rng is not None:
+            state["rng"] = rng.bit_generator.state
+        np.savez(path, **{k: v.detach().cpu().numpy() for k, v in self.p.items()},
+                 __adam_m__=self.m.cpu().numpy(), __adam_v__=self.v.cpu().numpy(),
+                 __state__=np.array(json.dumps(state)), __meta__=np.array(json.dumps(meta)))
+
+    def load_checkpoint(self, path):
+        """Restore weights, AdamW moments and step count from save_checkpoint;
+        returns a NumPy Generator in the saved state (or None)."""
+        with np.load(path, allow_pickle=False) as f:
+            self.load_params({k: f[k] for k in f.files if not k.startswith("__")})
+            self.m.copy_(torch.from_numpy(f["__adam_m__"]))
+            self.v.copy_(torch.from_numpy(f["__adam_v__"]))
+            state = json.loads(str(f["__state__"]))
+        self.step_count = int(state["step_count"])
+        self.pack()
+        if "rng" not in state:
+            return None
+        rng = np.random.Generator(np.random.PCG64())
+        rng.bit_generator.state = state["rng"]
+        return rng
+
     @staticmethod
     def load_file(path):
         with np.load(path, allow_pickle=False) as f:
-            params = {k: f[k].astype(np.float64) for k in f.files if k != "__meta__"}
+            params = {k: f[k].astype(np.float64) for k in f.files if not k.startswith("__")}
             meta = json.loads(str(f["__meta__"])) if "__meta__" in f.files else {}
         return params, meta
 
@@ -426,26 +454,28 @@ def _symmetry(delta, target, rng):
 
 
 def train(model, boxes, steps, lr=2e-3, lr_min=1e-4, seed=0, log_every=50, log=None, time_budget=None,
-          augment=False):
+          augment=False, rng=None, start=0, checkpoint=None, checkpoint_every=0):
     """AdamW with a cosine learning-rate schedule over random batches of the
     BoxSet (order from NumPy PCG64(seed)), optionally with a random symmetry
     of the periodic cube per batch.  With time_budget (seconds) the number of
-    steps is cut to what fits, measured on the first 10 steps.  Returns the
-    logged (step, loss) pairs."""
+    steps is cut to what fits, measured on the first 10 steps.  A resumed run
+    passes the generator load_checkpoint returned and the step to start from;
+    with checkpoint (a path) a checkpoint is written every checkpoint_every
+    steps and at the end.  Returns the logged (step, loss) pairs."""
     import time
-    rng = np.random.Generator(np.random.PCG64(int(seed)))
+    rng = rng if rng is not None else np.random.Generator(np.random.PCG64(int(seed)))
     losses = []
     n = len(boxes)
     t0 = None
-    it = 0
+    it = int(start)
     while it < int(steps):
-        if time_budget and it == 10:
+        if time_budget and it == start + 10:
             torch.cuda.synchronize()
             per = (time.perf_counter() - t0) / 10
-            steps = min(int(steps), 10 + int((time_budget - 10 * per) / per))
+            steps = min(int(steps), start + 10 + int((time_budget - 10 * per) / per))
             if log:
                 log(f"{per * 1e3:.1f} ms/step: training for {steps} steps")
-        if it == 0:
+        if it == start:
             torch.cuda.synchronize()
             t0 = time.perf_counter()
         sel = rng.choice(n, size=model.B, replace=False)
@@ -462,6 +492,10 @@ def train(model, boxes, steps, lr=2e-3, lr_min=1e-4, seed=0, log_every=50, log=N
             if log:
                 log(f"step {it} loss {lv:.5f} lr {cur:.2e}")
         it += 1
+        if checkpoint and checkpoint_every and it % checkpoint_every == 0:
+            model.save_checkpoint(checkpoint, rng=rng, step=it, steps=int(steps))
+    if checkpoint:
+        model.save_checkpoint(checkpoint, rng=rng, step=it, steps=int(steps))
     return losses
 
 

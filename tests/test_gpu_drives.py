@@ -38,6 +38,22 @@ def test_zscroll_matches_stage_by_stage(gpu):
         assert len(ps) == 40 and np.all(np.isfinite(ps))
     # later redshifts are more ionized
     assert out["x_HI"][0] >= out["x_HI"][1] >= out["x_HI"][2]
+    # the lowest redshift against the FP64 oracle on the scroll's own density
+    from oracle import bubbles, sigma
+    from scipy.special import erfc
+    sp = pkg("cosmo_files.sigmas").PowerLawSpectrum(D, L)
+    z, dens = float(out["z"][-1]), out["density"][-1].cpu().numpy()
+    dc = 1.686 / bm.linear_growth(z)
+    radii, _ = sigma.radius_schedule(L, D)
+    den = sigma.denominators(radii, 40.0, sp.sig0_discrete)
+    fcm = float(np.float32(erfc(dc / np.sqrt(80.0))))
+    want, margin = bubbles.find_bubbles_f64(dens, L, radii, den, 30.0, dc, LE=False,
+                                            growth_inv=float(np.float32(dc / 1.686)), fc_mean=fcm)
+    got = out["ionized"][-1].cpu().numpy()
+    ok = margin > 1e-5
+    assert np.array_equal((got == 1)[ok], (want == 1)[ok])
+    part = ok & (got < 1) & (want < 1)
+    assert np.allclose(got[part], want[part], rtol=1e-4, atol=1e-7)
 
 
 def test_zscroll_planck_mode(gpu):

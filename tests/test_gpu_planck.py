@@ -75,6 +75,13 @@ def test_velocities_at_z_are_growth_rate_times_initial(gpu):
     ics = init.init(DIM=DIM, HII_DIM=HII, BOX_LEN=L, seed=5, save=False)
     f = pl.ddt_growth_over_growth(z)
     vs = pf.velocities_at(z, ics["deltax"], HII_DIM=HII, box_len=L)
+    # against the oracle's initialize velocities on the same spectrum table
+    from oracle import grf
+    want = grf.initialize(DIM, HII, L, 5, pk_table=ics["pk_table"])["v"]
+    for axis, (v, w) in enumerate(zip(vs, want)):
+        ref = w.astype(np.float64) * f
+        err = float(np.abs(v.double().cpu().numpy() - ref).max())
+        assert err <= 1e-4 * float(np.abs(ref).max()), axis
     for axis, (v, v0) in enumerate(zip(vs, ics["v"])):
         ref = (v0.double() * f).cpu()
         err = float((v.double().cpu() - ref).abs().max())

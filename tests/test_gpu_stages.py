@@ -89,3 +89,30 @@ def test_stage_chain_through_files(gpu, tmp_path):
     _, want_ps, cnt, _ = deltat.power_spectrum(want_dT.astype(np.float64), L, edges)
     live = cnt > 0
     assert np.allclose(ps[live], want_ps[live], rtol=1e-4, atol=1e-12)
+
+
+def test_linear_evolution_and_raw_box_input(gpu, tmp_path):
+    """perturb_field.run(z) with EVOLVE_DENSITY_LINEARLY: the smoothed z=0
+    box times the Planck13 D(z) (perturb_field.py:39-50, 133-138), written
+    under the reference's name; find_HII_bubbles.run on the same box stored
+    as a raw 21cmFAST float32 file (boxio) gives the result of the .npy
+    input."""
+    init = pkg("Programs.initialize")
+    pf = pkg("Programs.perturb_field")
+    fb = pkg("Programs.find_HII_bubbles")
+    pl = pkg("cosmo_files.planck")
+    boxio = pkg("tocmfastpy.boxio")
+    io = str(tmp_path)
+    boxes = os.path.join(io, "Boxes")
+    init.run(DIM=DIM, HII_DIM=HII, BOX_LEN=L, seed=3, out_dir=io, with_velocity=False)
+    upd = pf.run(Z, DIM=DIM, HII_DIM=HII, BOX_LEN=L, io_dir=io, linear=True).cpu().numpy()
+    sm = np.load(os.path.join(boxes, f"smoothed_deltax_z0.00_{HII}_{L:.0f}Mpc.npy"))
+    assert np.array_equal(upd, perturb.evolve_linear(sm, pl.fgrowth(Z)))
+    name = os.path.join(boxes, f"updated_smoothed_deltax_z{Z:.2f}_{HII}_{L:.0f}Mpc")
+    assert np.array_equal(np.load(name + ".npy"), upd)
+    # the raw file of the same box (no extension): the reader path of find_HII_bubbles.run
+    boxio.save_box(name, upd)
+    opts = types.SimpleNamespace(DIR=boxes, FILTER_TYPE=1, UPDATE_TYPE=0, Z=Z, QUIET=True)
+    a = fb.run(opts, [name + ".npy"])
+    b = fb.run(opts, [name])
+    assert np.array_equal(a, b) and 0.0 < np.mean(a == 1) < 1.0
