@@ -119,6 +119,7 @@ SIGNATURES = {
     "cm21_update_sphere": (I, [VP, VP, I, F, F, D, VP, F, F, I, VP, VP]),
     "cm21_find_bubbles_workspace_bytes": (SZ, [I, I, I]),
     "cm21_find_bubbles": (I, [VP, VP, VP, c_float_p, c_float_p, I, ctypes.POINTER(BubbleParams), VP, SZ, VP]),
+    "cm21_find_bubbles_noalloc": (I, [VP, VP, VP, c_float_p, c_float_p, I, ctypes.POINTER(BubbleParams), VP]),
     "cm21_grf_init": (I, [VP, I, D, D, D, D, U32, U32, VP]),
     "cm21_grf_init_table": (I, [VP, I, D, VP, VP, I, U32, U32, VP]),
     "cm21_subsample": (I, [VP, VP, I, I, F, VP]),

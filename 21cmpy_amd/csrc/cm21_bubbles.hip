@@ -496,6 +496,58 @@ int graph_loop(cm21_plan* plan, const float* delta, float* ionized, const float*
 }
 }  // namespace
 
+// ------------------------------------------------------------------ no-alloc loop
+// cm21_find_bubbles_noalloc runs the loop of cm21_find_bubbles with its
+// workspace in a file-scope device array instead of a caller's buffer.  Each
+// GPU holds its own copy of the array (CM21_NOALLOC_SCRATCH_BYTES) from the
+// library's load until the process ends; host code reaches it only through
+// the address hipGetSymbolAddress returns on that device, taken once per
+// device at the first call.  Calls on one GPU share the array.
+namespace {
+constexpr size_t kScratchBytes = (size_t)CM21_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+int scratch_on_current_device(void** out) {
+    int dev = 0;
+    CM21_HIP_CHECK(hipGetDevice(&dev));
+    CM21_REQUIRE(dev >= 0 && dev < kMaxDevices, "cm21_find_bubbles_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return cm21::fail(CM21_ERR_HIP, "cm21_find_bubbles_noalloc: hipGetSymbolAddress(scratch) on device %d: %s",
+                              dev, hipGetErrorString(e));
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return CM21_OK;
+}
+}  // namespace
+
+extern "C" int cm21_find_bubbles_noalloc(cm21_plan* plan, const float* delta, float* ionized, const float* radii,
+                                         const float* denoms, int n_radii, const cm21_bubble_params* prm,
+                                         void* stream) {
+    CM21_TRY(check_loop_args("cm21_find_bubbles_noalloc", plan, delta, ionized, radii, denoms, n_radii, prm));
+    const int D = plan_dim(plan);
+    const size_t need = carve(nullptr, D, n_radii, prm->update).bytes;
+    if (need > kScratchBytes)
+        return cm21::fail(CM21_ERR_NOMEM,
+                          "cm21_find_bubbles_noalloc: D=%d with %d radii (update %d) needs %zu bytes of workspace, "
+                          "more than the %zu-byte static scratch; use cm21_find_bubbles with a caller workspace",
+                          D, n_radii, prm->update, need, kScratchBytes);
+    void* base = nullptr;
+    CM21_TRY(scratch_on_current_device(&base));
+    LoopWork w = carve(base, D, n_radii, prm->update);
+    // the launches go straight to the stream (params->graph is ignored), so a
+    // caller that captures the stream records them into its own graph
+    return run_loop(plan, delta, ionized, radii, denoms, n_radii, prm, w, (hipStream_t)stream);
+}
+
 namespace cm21 {
 void forget_plan_graphs(const cm21_plan* plan) {
     std::lock_guard<std::mutex> lk(g_graph_mu);

@@ -140,6 +140,23 @@ int cm21_find_bubbles(cm21_plan* plan, const float* delta, float* ionized,
                       const cm21_bubble_params* params, void* work,
                       size_t work_bytes, void* stream);
 
+/* No counterpart in the reference: a variant of cm21_find_bubbles for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It allocates nothing: the
+ * workspace is a static device array of CM21_NOALLOC_SCRATCH_BYTES that the
+ * library holds on each GPU it runs on until the process ends, and the call
+ * makes no hipMalloc, hipFree or allocating library call (the plan, created
+ * beforehand, brings its own rocFFT work buffer).  Boxes whose workspace
+ * (cm21_find_bubbles_workspace_bytes) exceeds the array are rejected with
+ * CM21_ERR_NOMEM: every mode fits for D <= 128.  params->graph is ignored (the
+ * launches go to `stream`, where a capturing caller records them).  Calls on
+ * one GPU share the array: calls that overlap in time on one GPU race, so
+ * order them on one stream. */
+#define CM21_NOALLOC_SCRATCH_BYTES (64u << 20)
+int cm21_find_bubbles_noalloc(cm21_plan* plan, const float* delta, float* ionized,
+                              const float* radii, const float* denoms, int n_radii,
+                              const cm21_bubble_params* params, void* stream);
+
 /* ------------------------------------------------------------------ init
  * Replaces init_kernel + MRG32k3a normals + complex assembly
  * (Programs/initialize.cu:58-95; initialize.py:63-72): writes the c2r-ready
