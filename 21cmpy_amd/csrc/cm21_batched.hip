@@ -140,7 +140,7 @@ __global__ __launch_bounds__(L* N1) void k_batch_lines_sum(BatchLaunch a) {
 #pragma unroll 4
     for (int f = t; f < L * D; f += NT) {
         const int i = f / D, rr = f - i * D;
-        acc += (double)fcoll_for_sum(xrf[i * 2 * C::HP + rr], cell.deltac, inv_denom);
+        acc += (double)fcoll_for_sum(xrf[i * 2 * C::HP + rr], cell.deltac, cell.denom, inv_denom);
     }
     acc = block_sum(acc);
     if (t == 0) a.partial[((size_t)box * a.nr + r) * a.nblk + blockIdx.x] = acc;

@@ -247,6 +247,8 @@ struct LoopWork {
     double* lpart;
     float* bmax;
     ScaledThreshold* scaled;
+    int* active;   // line groups the threshold pass visits, and their number
+    int* nactive;
     size_t bytes;
 };
 
@@ -278,6 +280,10 @@ LoopWork carve(void* base, int D, int n_radii, int update) {
     off = align_up(off + sizeof(float) * (fused_partials(D) + 1), 256);
     w.scaled = (ScaledThreshold*)(p + off);
     off = align_up(off + sizeof(ScaledThreshold) * (size_t)(n_radii > 0 ? n_radii : 1), 256);
+    w.active = (int*)(p + off);
+    off = align_up(off + sizeof(int) * (fused_partials(D) + 1), 256);
+    w.nactive = (int*)(p + off);
+    off = align_up(off + sizeof(int), 256);
     w.bytes = off;
     return w;
 }
@@ -388,15 +394,20 @@ int run_loop(cm21_plan* plan, const float* delta, float* ionized, const float* r
                                                 ionized, w.lpart, (double)N, w.bmax);
                 CM21_TRY(fused_lines(*sh, 0, T[slot], tw, fp, es, st));
                 CM21_TRY(fused_sum(w.lpart, (int)fused_partials(D), w.sums + r, st, w.scaled + r, prm->fc_mean,
-                                   (double)N, prm->deltac, denoms[r], prm->zeta));
+                                   (double)N, prm->deltac, denoms[r], prm->zeta, w.bmax, w.active, w.nactive));
                 sum = w.sums + r;
             }
             LineEpilogue ep = line_epilogue(prm->deltac, denoms[r], prm->zeta, sum, prm->fc_mean, w.mask, ionized,
                                             w.lpart, (double)N, prm->linear_input ? nullptr : w.bmax);
-            if (prm->linear_input)
+            if (prm->linear_input) {
                 ep.th = density_threshold_host(prm->deltac, denoms[r], prm->zeta, 1.0f);
-            else
+            } else {
                 ep.scaled = w.scaled + r;
+                if (!final_step) {
+                    ep.active = w.active;
+                    ep.nactive = w.nactive;
+                }
+            }
             CM21_TRY(fused_lines(*sh, final_step ? 2 : 1, T[slot], tw, fp, ep, st,
                                  prm->linear_input ? CM21_STAGE_FUSED_LINES : CM21_STAGE_FUSED_LINES_THRESHOLD));
             CM21_HIP_CHECK(hipEventRecord(lines_done[slot], st));

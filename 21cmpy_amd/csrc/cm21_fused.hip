@@ -22,6 +22,8 @@
 // integer bound, except in a thin shell around the cut where the reference's
 // float32 test (find_bubbles.cu:26-35) decides, so cut decisions are
 // bit-identical to the reference arithmetic.
+#include <algorithm>
+
 #include "cm21_fused_dev.hpp"
 
 #pragma clang fp contract(fast)
@@ -58,16 +60,15 @@ __global__ __launch_bounds__(W* D1) void k_fused_rows(float2* __restrict__ T, co
 // found its maximum below d* - band (no cell can reach the threshold; cells
 // inside the band take the exact test, so the skip never drops a hit).  A
 // final-paint block whose cells are all ionized writes 1 without transforming.
+// `blk` is the index of the L-line group the block works on.
 template <int N1, int N2, int L, int MODE>
-__global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict__ U, const float2* __restrict__ tw,
-                                                      FusedParams fp, LineEpilogue ep) {
+__device__ __forceinline__ void lines_block(const float2* __restrict__ U, const float2* __restrict__ tw,
+                                            const FusedParams& fp, const LineEpilogue& ep, int blk, float2* buf,
+                                            float2* stw, unsigned long long* smask) {
     using C = LinesCore<N1, N2, L>;
     constexpr int D = C::D, H = C::H, NT = C::NT, WORDS = C::WORDS;
-    __shared__ float2 buf[L * C::HP];
-    __shared__ float2 stw[D];
-    __shared__ unsigned long long smask[WORDS];
     const int t = threadIdx.x;
-    const size_t line0 = (size_t)blockIdx.x * L;
+    const size_t line0 = (size_t)blk * L;
     const size_t word0 = line0 * D / 64;
     const size_t cell0 = line0 * D;
     const bool use_scale = ep.scaled != nullptr;
@@ -87,7 +88,7 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
                 for (int q = t; q < L * D; q += NT) ep.out[cell0 + q] = 1.0f;
             return;
         }
-        if (MODE == kLineThreshold && ep.bmax != nullptr && ep.bmax[blockIdx.x] < th.dstar - th.band) return;
+        if (MODE == kLineThreshold && ep.bmax != nullptr && ep.bmax[blk] < th.dstar - th.band) return;
     }
     // all global loads in flight before the first LDS store
     float2 tmp[C::LOADS];
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
         const int i = f / D, r = f - i * D;
         const float delta = xrf[i * 2 * C::HP + r];
         if (MODE == kLineSum) {
-            acc += (double)fcoll_for_sum(delta, ep.deltac, inv_denom);
+            acc += (double)fcoll_for_sum(delta, ep.deltac, ep.denom, inv_denom);
             vmax = fmaxf(vmax, delta);
         } else {
             const bool hit = cell_hit(delta, th, ep.deltac, ep.denom, use_scale, scale, ep.inv_zeta);
@@ -133,16 +134,36 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
             if (t == 0) {
                 float m = wmax[0];
                 for (int w = 1; w < NT / 64; ++w) m = fmaxf(m, wmax[w]);
-                ep.bmax[blockIdx.x] = m;
+                ep.bmax[blk] = m;
             }
         }
         acc = block_sum(acc);
-        if (t == 0) ep.partial[blockIdx.x] = acc;
+        if (t == 0) ep.partial[blk] = acc;
     }
     if (MODE == kLineThreshold) {
         __syncthreads();
         if (t < WORDS && smask[t] != mword) ep.mask[word0 + t] = smask[t];
     }
+}
+
+// With an active list (threshold pass after a box sum) block b works on the
+// b-th listed line group and the blocks past the end of the list return at
+// once, so line groups that cannot change cost one scalar load each and the
+// groups that can are spread over the first blocks of the dispatch;
+// otherwise block b works on line group b.
+template <int N1, int N2, int L, int MODE>
+__global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict__ U, const float2* __restrict__ tw,
+                                                      FusedParams fp, LineEpilogue ep) {
+    using C = LinesCore<N1, N2, L>;
+    __shared__ float2 buf[L * C::HP];
+    __shared__ float2 stw[C::D];
+    __shared__ unsigned long long smask[C::WORDS];
+    int blk = blockIdx.x;
+    if (MODE == kLineThreshold && ep.active != nullptr) {
+        if (blk >= *ep.nactive) return;
+        blk = ep.active[blk];
+    }
+    lines_block<N1, N2, L, MODE>(U, tw, fp, ep, blk, buf, stw, smask);
 }
 
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, size_t n) {
@@ -161,6 +182,9 @@ struct SumScaleArgs {
     ScaledThreshold* scaled;
     float fc_mean, deltac, denom;
     double ncells, zeta;
+    const float* bmax;  // with `active`: per-group maxima of the box-sum pass
+    int* active;        // receives the line groups that can reach the threshold
+    int* nactive;
 };
 
 __global__ __launch_bounds__(kSumThreads) void k_sum_partials_lines(const double* __restrict__ partial, int n,
@@ -171,6 +195,8 @@ __global__ __launch_bounds__(kSumThreads) void k_sum_partials_lines(const double
     acc = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
     __syncthreads();
+    __shared__ DensityThreshold sth;
+    __shared__ int count;
     if (threadIdx.x == 0) {
         double r = 0.0;
         for (int w = 0; w < kSumThreads / 64; ++w) r += part[w];
@@ -182,8 +208,20 @@ __global__ __launch_bounds__(kSumThreads) void k_sum_partials_lines(const double
             s.th = density_threshold(sa.deltac, sa.denom, sa.zeta, scale);
             s.pad = 0.0f;
             *sa.scaled = s;
+            sth = s.th;
         }
+        count = 0;
     }
+    if (!sa.scaled || !sa.active) return;
+    __syncthreads();
+    // the line groups the threshold pass must visit: those whose maximum is
+    // not below d* - band (the order in the list does not matter: each group
+    // writes only its own mask words)
+    const float lim = sth.dstar - sth.band;
+    for (int i = threadIdx.x; i < n; i += kSumThreads)
+        if (!(sa.bmax[i] < lim)) sa.active[atomicAdd(&count, 1)] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) *sa.nactive = count;
 }
 
 // ------------------------------------------------------------------ host side
@@ -355,8 +393,8 @@ LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double*
 }
 
 int fused_sum(const double* partial, int n, double* out, hipStream_t st, ScaledThreshold* scaled, float fc_mean,
-              double ncells, float deltac, float denom, double zeta) {
-    SumScaleArgs sa{scaled, fc_mean, deltac, denom, ncells, zeta};
+              double ncells, float deltac, float denom, double zeta, const float* bmax, int* active, int* nactive) {
+    SumScaleArgs sa{scaled, fc_mean, deltac, denom, ncells, zeta, bmax, active, nactive};
     k_sum_partials_lines<<<1, kSumThreads, 0, st>>>(partial, n, out, sa);
     CM21_LAUNCH_CHECK("k_sum_partials_lines");
     return CM21_OK;

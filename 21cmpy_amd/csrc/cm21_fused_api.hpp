@@ -51,6 +51,8 @@ struct LineEpilogue {
                                // blocks no cell of which can reach the threshold; or NULL
     DensityThreshold th;       // threshold when `scaled` is NULL (linear input)
     const ScaledThreshold* scaled;  // nonlinear input: scale and threshold of the radius
+    const int* active;         // threshold pass: the line groups to visit, or NULL (all)
+    const int* nactive;        //   and their number
 };
 
 struct FusedShape {
@@ -74,8 +76,10 @@ int fused_lines(const FusedShape& sh, int mode, const float2* U, const float2* t
 int fused_mask_clear(unsigned long long* mask, size_t words, hipStream_t st);
 // Fixed-order FP64 sum of the box-sum partials into *out; with `scaled`
 // also the box-mean scale float32(fc_mean / (sum/ncells)) and its threshold.
+// With bmax, active and nactive it also lists the line groups whose maximum
+// can reach the threshold (for the threshold pass).
 int fused_sum(const double* partial, int n, double* out, hipStream_t st, ScaledThreshold* scaled = nullptr,
               float fc_mean = 0.0f, double ncells = 1.0, float deltac = 0.0f, float denom = 1.0f,
-              double zeta = 1.0);
+              double zeta = 1.0, const float* bmax = nullptr, int* active = nullptr, int* nactive = nullptr);
 
 }  // namespace cm21

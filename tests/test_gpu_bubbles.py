@@ -244,3 +244,30 @@ def test_graph_replay_matches_eager(gpu, LE):
         fb.conv_bubbles(other, {"BoxSize": 100.0, "HIIdim": 64}, LE=True, quiet=True, deltac=1.686, sigma_min2=0.5,
                         zeta=20.0)
         torch.cuda.synchronize()
+
+
+def test_nonlinear_constant_box_closed_form(gpu):
+    """Nonlinear input on a constant box: the box-mean normalisation makes
+    f_coll * scale equal fc_mean in every cell at every radius
+    (find_HII_bubbles.py:158-161,171-174), so with zeta fc_mean < 1 nothing
+    ionizes and the final paint is zeta fc_mean everywhere, whatever the
+    density.  The sweep puts the erfc argument of the last radius from -9 to
+    9: the box sum (erfc_sum) and the decision path (erfcf) must agree to a
+    few float32 ulp there, or the paint moves off zeta fc_mean."""
+    fb = pkg("Programs.find_HII_bubbles")
+    D, L, deltac, smin, zeta = 64, 100.0, 3.0, 1.0, 30.0
+    sp = pkg("cosmo_files.sigmas").PowerLawSpectrum(D, L)
+    radii, den, fc_mean = fb.bubble_setup(D, L, None, deltac, smin, sp.sig0_discrete)
+    g = float(np.float32(deltac / 1.686))
+    target = zeta * float(fc_mean)
+    assert target < 0.9
+    worst = 0.0
+    for x in np.linspace(-9.0, 9.0, 37):
+        delta_lin = deltac - x * float(den[-1])
+        box = torch.full((D, D, D), delta_lin / g, dtype=torch.float32, device="cuda")
+        got = fb.conv_bubbles(box, {"BoxSize": L, "HIIdim": D}, LE=False, quiet=True, deltac=deltac,
+                              sigma_min2=smin, zeta=zeta, sig0=sp.sig0_discrete)
+        lo, hi = float(got.min()), float(got.max())
+        worst = max(worst, abs(lo / target - 1), abs(hi / target - 1))
+        assert abs(lo / target - 1) < 1.5e-6 and abs(hi / target - 1) < 1.5e-6, (x, lo, hi, target)
+    print(f"worst relative deviation of the paint from zeta fc_mean: {worst:.2e}", flush=True)
