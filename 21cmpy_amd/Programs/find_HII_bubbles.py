@@ -28,7 +28,7 @@ from scipy.special import erfc
 from .. import _lib
 from ..Parameter_files import INIT_PARAMS as P
 from ..cosmo_files.planck import barrier as planck_barrier
-from ..cosmo_files.sigmas import PowerLawSpectrum
+from ..cosmo_files.sigmas import power_law_spectrum
 
 DELTA_C_LINEAR = 1.686
 
@@ -78,7 +78,7 @@ def barrier_denominators(radii, sigma_min2, sig0_fn):
 
 @functools.lru_cache(maxsize=16)
 def _default_spectrum(D, box_len):
-    return PowerLawSpectrum(D, box_len)
+    return power_law_spectrum(D, box_len)
 
 
 def _default_sig0(D, box_len):

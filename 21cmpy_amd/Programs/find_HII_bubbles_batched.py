@@ -24,7 +24,7 @@ from scipy.special import erfc
 
 from .. import _lib
 from ..cosmo_files.planck import barrier as planck_barrier
-from ..cosmo_files.sigmas import PowerLawSpectrum
+from ..cosmo_files.sigmas import power_law_spectrum
 from .find_HII_bubbles import DELTA_C_LINEAR, barrier_denominators, radius_schedule
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -131,7 +131,7 @@ class BatchedBubbles:
         for (D, L), idx in order.items():
             radii, _ = radius_schedule(L, D)
             if (D, L) not in sig_cache:
-                sig_cache[(D, L)] = PowerLawSpectrum(D, L).sig0_discrete
+                sig_cache[(D, L)] = power_law_spectrum(D, L).sig0_discrete
             denoms = np.empty((len(idx), len(radii)), np.float32)
             boxes = (_lib.BatchBox * len(idx))()
             for j, i in enumerate(idx):

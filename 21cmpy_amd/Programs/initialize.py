@@ -25,7 +25,7 @@ import torch
 from .. import gpu, _lib
 from ..Parameter_files import INIT_PARAMS as P
 from ..cosmo_files import planck
-from ..cosmo_files.sigmas import PowerLawSpectrum
+from ..cosmo_files.sigmas import power_law_spectrum
 
 BLOCK_VOLUME = 512  # points of the P(k) table (initialize.py:28)
 
@@ -67,7 +67,7 @@ def init(DIM=None, HII_DIM=None, BOX_LEN=None, seed=0, box_id=0, pspec="planck",
         raise ValueError(f"HII_DIM={HII_DIM} must divide DIM={DIM}")
     table = None
     if isinstance(pspec, str) and pspec == "powerlaw":
-        spectrum = PowerLawSpectrum(DIM, L, sigma_cell2, n_index)
+        spectrum = power_law_spectrum(DIM, L, sigma_cell2, n_index)
         spec = gpu.grf_init(DIM, L, spectrum.amp, seed, box_id, n_index, spectrum.k_cut)
     else:
         if isinstance(pspec, str) and pspec != "planck":

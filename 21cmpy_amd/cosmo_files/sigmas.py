@@ -15,6 +15,8 @@ the box's own modes carry a given per-cell variance:
   between the limits where Delta^2 W^2 exceeds 1e-6 of its maximum on a
   0.1-spaced ln k grid (sigmas.py:54-66,99-102,176-191).
 """
+import functools
+
 import numpy as np
 
 
@@ -107,6 +109,13 @@ class PowerLawSpectrum:
         if method == "trapz":
             return self.sig0_trapz(R)
         raise ValueError(f"unknown sig0 method {method!r}")
+
+
+@functools.lru_cache(maxsize=32)
+def power_law_spectrum(D, box_len, sigma_cell2=0.36, n_index=-2.5):
+    """A shared PowerLawSpectrum per (D, box_len, sigma_cell2, n_index): its
+    normalisation and its S(R) values are computed once per process."""
+    return PowerLawSpectrum(int(D), float(box_len), float(sigma_cell2), float(n_index))
 
 
 def sig0_trapz(R, del2_fn, nint=1000, factor=1.0e-6):

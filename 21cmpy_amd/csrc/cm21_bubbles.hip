@@ -9,6 +9,7 @@
 //    input are folded into the filter multiply;
 //  * the box mean of f_coll is an FP64 device scalar read by the threshold
 //    kernel, so the loop never synchronises with the host.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -23,15 +24,13 @@ using namespace cm21;
 
 namespace {
 
-__global__ void k_filter(float2* __restrict__ out, const float2* __restrict__ in, int D, size_t Nc,
-                         WindowParams wp, float scale) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= Nc) return;
+__global__ void k_filter(float2* __restrict__ out, const float2* __restrict__ in, int D, WindowParams wp,
+                         float scale) {
     const int H = D / 2 + 1;
-    const int c = (int)(idx % H);
-    const size_t ab = idx / H;
-    const int b = (int)(ab % D);
-    const int a = (int)(ab / D);
+    const int c = blockIdx.x * kModeBlockX + threadIdx.x, b = blockIdx.y * kModeBlockY + threadIdx.y;
+    const int a = blockIdx.z;
+    if (c >= H || b >= D) return;
+    const size_t idx = ((size_t)a * D + b) * H + c;
     float w0, w2;
     bool cut;
     window_of(kmag_of(a, b, c, D, wp.dk, wp.korder), wp, w0, cut, w2);
@@ -136,8 +135,8 @@ __global__ void k_sphere_apply(float* __restrict__ ion, const uint32_t* __restri
 
 int launch_filter(void* spec_out, const void* spec_in, int D, const WindowParams& wp, float scale,
                   hipStream_t st) {
-    const size_t Nc = (size_t)D * D * (D / 2 + 1);
-    k_filter<<<grid_for(Nc), kThreads, 0, st>>>((float2*)spec_out, (const float2*)spec_in, D, Nc, wp, scale);
+    k_filter<<<mode_grid(D, D / 2 + 1), mode_block(), 0, st>>>((float2*)spec_out, (const float2*)spec_in, D, wp,
+                                                               scale);
     CM21_LAUNCH_CHECK("k_filter");
     return CM21_OK;
 }
@@ -238,17 +237,18 @@ namespace {
 struct LoopWork {
     float2* spec;
     float2* filt;
-    float2* filt2;  // second intermediate of the fused loop (radii alternate)
+    float2* filt2;  // second and third intermediates of the fused loop
+    float2* filt3;  //   (radius r uses intermediate r % 3)
     float* smoothed;
     double* partial;
     double* sums;
     void* sphere;
     unsigned long long* mask;
-    double* lpart;
-    float* bmax;
+    double* lpart[2];  // box-sum partials, per radius parity
+    float* bmax[2];    // per line group maxima, per radius parity
     ScaledThreshold* scaled;
-    int* active;   // line groups the threshold pass visits, and their number
-    int* nactive;
+    int* active[2];    // line groups the threshold pass visits, and their number
+    int* nactive[2];
     size_t bytes;
 };
 
@@ -264,8 +264,11 @@ LoopWork carve(void* base, int D, int n_radii, int update) {
     off = align_up(off + sizeof(float2) * Nc, 256);
     w.filt2 = (float2*)(p + off);
     off = align_up(off + sizeof(float2) * Nc, 256);
+    // the generic path's smoothed box and the fused path's third
+    // intermediate share one region (8Nc >= 4N bytes)
     w.smoothed = (float*)(p + off);
-    off = align_up(off + sizeof(float) * N, 256);
+    w.filt3 = (float2*)(p + off);
+    off = align_up(off + std::max(sizeof(float) * N, sizeof(float2) * Nc), 256);
     w.partial = (double*)(p + off);
     off = align_up(off + cm21_reduce_workspace_bytes(), 256);
     w.sums = (double*)(p + off);
@@ -274,16 +277,18 @@ LoopWork carve(void* base, int D, int n_radii, int update) {
     if (update) off = align_up(off + cm21_sphere_workspace_bytes(D), 256);
     w.mask = (unsigned long long*)(p + off);
     off = align_up(off + sizeof(unsigned long long) * ((N + 63) / 64), 256);
-    w.lpart = (double*)(p + off);
-    off = align_up(off + sizeof(double) * (fused_partials(D) + 1), 256);
-    w.bmax = (float*)(p + off);
-    off = align_up(off + sizeof(float) * (fused_partials(D) + 1), 256);
+    for (int k = 0; k < 2; ++k) {
+        w.lpart[k] = (double*)(p + off);
+        off = align_up(off + sizeof(double) * (fused_partials(D) + 1), 256);
+        w.bmax[k] = (float*)(p + off);
+        off = align_up(off + sizeof(float) * (fused_partials(D) + 1), 256);
+        w.active[k] = (int*)(p + off);
+        off = align_up(off + sizeof(int) * (fused_partials(D) + 1), 256);
+        w.nactive[k] = (int*)(p + off);
+        off = align_up(off + sizeof(int), 256);
+    }
     w.scaled = (ScaledThreshold*)(p + off);
     off = align_up(off + sizeof(ScaledThreshold) * (size_t)(n_radii > 0 ? n_radii : 1), 256);
-    w.active = (int*)(p + off);
-    off = align_up(off + sizeof(int) * (fused_partials(D) + 1), 256);
-    w.nactive = (int*)(p + off);
-    off = align_up(off + sizeof(int), 256);
     w.bytes = off;
     return w;
 }
@@ -357,61 +362,104 @@ int run_loop(cm21_plan* plan, const float* delta, float* ionized, const float* r
     if (prm->fused && sh && prm->update == 0 && (prm->filter_type == 1 || prm->filter_type == 2)) {
         const float2* tw = plan_twiddles(plan);
         CM21_TRY(fused_mask_clear(w.mask, (N + 63) / 64, st));
-        // The columns and rows passes of radius r+1 do not depend on the lines
-        // passes of radius r: they run on a side stream into the other
-        // intermediate buffer, joined by events (captured into the same graph
-        // when the caller's stream is capturing).
+        // Two streams, joined by events (captured into the same graph when
+        // the caller's stream is capturing).  The columns and rows passes of
+        // a radius depend only on the pristine spectrum and a free
+        // intermediate, so they run ahead on side stream ss into one of
+        // three intermediates (radius r uses T[r % 3]); the lines passes stay
+        // on the caller's stream st.
+        //
+        // Nonlinear input: the box-sum pass of radius r (c2r along the last
+        // axis, FP64 partials and per-group maxima) and the threshold pass of
+        // radius r-1 (which recomputes the last-axis transform, in the listed
+        // groups only, rather than storing and re-reading the smoothed field)
+        // form one launch; a one-block reduction then derives the box-mean
+        // scale and density threshold of radius r and lists the line groups
+        // whose maximum can reach it.  So st runs two launches per radius,
+        // and T[(r-1) % 3] is free once the first is done.
+        // Linear input: one threshold pass per radius, threshold from the
+        // host (scale 1); T[r % 3] is free after it.
         SideStreams* ov = nullptr;
         CM21_TRY(side_streams(&ov));
         std::lock_guard<std::mutex> lk(ov->mu);
         hipStream_t ss = ov->s[0];
-        hipEvent_t* rows_done = ov->ev;       // [2]
-        hipEvent_t* lines_done = ov->ev + 2;  // [2]
-        float2* T[2] = {w.filt, w.filt2};
-        CM21_HIP_CHECK(hipEventRecord(ov->fork, st));
-        CM21_HIP_CHECK(hipStreamWaitEvent(ss, ov->fork, 0));
-        for (int r = 0; r < n_radii; ++r) {
-            const bool final_step = r == n_radii - 1;
-            const int slot = r & 1;
-            FusedParams fp = fused_params(D, dk, radii[r], prm->filter_type, prm->ref_fallthrough, scale);
-            if (r >= 2) CM21_HIP_CHECK(hipStreamWaitEvent(ss, lines_done[slot], 0));
-            CM21_TRY(fused_radius(*sh, w.spec, T[slot], tw, fp, ss));
-            CM21_HIP_CHECK(hipEventRecord(rows_done[slot], ss));
-            CM21_HIP_CHECK(hipStreamWaitEvent(st, rows_done[slot], 0));
-            // Nonlinear input: a sum pass for the FP64 box mean, then the
-            // threshold pass recomputes the last-axis transform (only in the
-            // blocks that can change) rather than storing and re-reading the
-            // smoothed field.
-            // The sum pass also records each block's maximum of the smoothed
-            // field, which lets the threshold pass skip blocks that cannot
-            // reach the threshold.
-            // The box-mean scale and the density threshold are derived once
-            // per radius: by the reduction kernel for nonlinear input, on the
-            // host for linear input (scale 1).
-            const double* sum = nullptr;
-            if (!prm->linear_input) {
-                LineEpilogue es = line_epilogue(prm->deltac, denoms[r], prm->zeta, nullptr, prm->fc_mean, w.mask,
-                                                ionized, w.lpart, (double)N, w.bmax);
-                CM21_TRY(fused_lines(*sh, 0, T[slot], tw, fp, es, st));
-                CM21_TRY(fused_sum(w.lpart, (int)fused_partials(D), w.sums + r, st, w.scaled + r, prm->fc_mean,
-                                   (double)N, prm->deltac, denoms[r], prm->zeta, w.bmax, w.active, w.nactive));
-                sum = w.sums + r;
-            }
-            LineEpilogue ep = line_epilogue(prm->deltac, denoms[r], prm->zeta, sum, prm->fc_mean, w.mask, ionized,
-                                            w.lpart, (double)N, prm->linear_input ? nullptr : w.bmax);
+        hipEvent_t* rows_done = ov->ev;       // [3]
+        hipEvent_t* lines_done = ov->ev + 3;  // [3]
+        float2* T[3] = {w.filt, w.filt2, w.filt3};
+        auto params = [&](int r) {
+            return fused_params(D, dk, radii[r], prm->filter_type, prm->ref_fallthrough, scale);
+        };
+        auto issue_radius = [&](int r) -> int {
+            const int k = r % 3;
+            if (r >= 3) CM21_HIP_CHECK(hipStreamWaitEvent(ss, lines_done[k], 0));
+            CM21_TRY(fused_radius(*sh, w.spec, T[k], tw, params(r), ss));
+            CM21_HIP_CHECK(hipEventRecord(rows_done[k], ss));
+            return CM21_OK;
+        };
+        // box-sum epilogue of radius r (nonlinear input)
+        auto sum_epilogue = [&](int r) {
+            const int q = r & 1;
+            return line_epilogue(prm->deltac, denoms[r], prm->zeta, nullptr, prm->fc_mean, w.mask, ionized,
+                                 w.lpart[q], (double)N, w.bmax[q]);
+        };
+        // T[k] is free: issue the columns and rows of the radius that uses it next
+        auto release = [&](int r) -> int {
+            CM21_HIP_CHECK(hipEventRecord(lines_done[r % 3], st));
+            if (r + 3 < n_radii) CM21_TRY(issue_radius(r + 3));
+            return CM21_OK;
+        };
+        // threshold (or, at the last radius, final paint) epilogue of radius r
+        auto threshold_epilogue = [&](int r) {
+            const int q = r & 1;
+            LineEpilogue e;
             if (prm->linear_input) {
-                ep.th = density_threshold_host(prm->deltac, denoms[r], prm->zeta, 1.0f);
+                e = line_epilogue(prm->deltac, denoms[r], prm->zeta, nullptr, prm->fc_mean, w.mask, ionized,
+                                  w.lpart[q], (double)N, nullptr);
+                e.th = density_threshold_host(prm->deltac, denoms[r], prm->zeta, 1.0f);
             } else {
-                ep.scaled = w.scaled + r;
-                if (!final_step) {
-                    ep.active = w.active;
-                    ep.nactive = w.nactive;
+                e = line_epilogue(prm->deltac, denoms[r], prm->zeta, w.sums + r, prm->fc_mean, w.mask, ionized,
+                                  w.lpart[q], (double)N, w.bmax[q]);
+                e.scaled = w.scaled + r;
+                if (r < n_radii - 1) {
+                    e.active = w.active[q];
+                    e.nactive = w.nactive[q];
                 }
             }
-            CM21_TRY(fused_lines(*sh, final_step ? 2 : 1, T[slot], tw, fp, ep, st,
-                                 prm->linear_input ? CM21_STAGE_FUSED_LINES : CM21_STAGE_FUSED_LINES_THRESHOLD));
-            CM21_HIP_CHECK(hipEventRecord(lines_done[slot], st));
+            return e;
+        };
+        CM21_HIP_CHECK(hipEventRecord(ov->fork, st));
+        CM21_HIP_CHECK(hipStreamWaitEvent(ss, ov->fork, 0));
+        for (int r = 0; r < n_radii && r < 3; ++r) CM21_TRY(issue_radius(r));
+        for (int r = 0; r < n_radii; ++r) {
+            const int k = r % 3;
+            CM21_HIP_CHECK(hipStreamWaitEvent(st, rows_done[k], 0));
+            if (prm->linear_input) {
+                CM21_TRY(fused_lines(*sh, r == n_radii - 1 ? 2 : 1, T[k], tw, params(r), threshold_epilogue(r), st));
+                CM21_TRY(release(r));
+                continue;
+            }
+            if (r == 0) {
+                CM21_TRY(fused_lines(*sh, 0, T[k], tw, params(r), sum_epilogue(r), st));
+            } else {
+                CM21_TRY(fused_lines_pair(*sh, T[(r - 1) % 3], params(r - 1), threshold_epilogue(r - 1), T[k],
+                                          params(r), sum_epilogue(r), tw, st));
+                CM21_TRY(release(r - 1));
+            }
+            const int q = r & 1;
+            const bool last = r == n_radii - 1;
+            CM21_TRY(fused_sum(w.lpart[q], (int)fused_partials(D), w.sums + r, st, w.scaled + r, prm->fc_mean,
+                               (double)N, prm->deltac, denoms[r], prm->zeta, w.bmax[q], last ? nullptr : w.active[q],
+                               last ? nullptr : w.nactive[q]));
         }
+        if (!prm->linear_input) {
+            const int r = n_radii - 1;
+            CM21_TRY(fused_lines(*sh, 2, T[r % 3], tw, params(r), threshold_epilogue(r), st,
+                                 CM21_STAGE_FUSED_LINES_THRESHOLD));
+        }
+        // join the side stream back (a capture requires every forked stream
+        // to be joined by the origin stream)
+        CM21_HIP_CHECK(hipEventRecord(ov->join[0], ss));
+        CM21_HIP_CHECK(hipStreamWaitEvent(st, ov->join[0], 0));
         return CM21_OK;
     }
     k_fill<<<grid_for(N), kThreads, 0, st>>>(ionized, N, 0.0f);

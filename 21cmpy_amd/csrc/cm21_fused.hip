@@ -48,6 +48,57 @@ __global__ __launch_bounds__(W* D1) void k_fused_rows(float2* __restrict__ T, co
     rows_body<D1, D2, W>(T, D / 2 + 1, tw, fp, blockIdx.y, blockIdx.x, lds);
 }
 
+// Box-sum reduction of one radius by the NT threads of one block: strided
+// per-thread FP64 sums, the wavefront tree, then the waves in order (the order
+// is fixed, so the total is bitwise reproducible).  With `scaled` thread 0
+// also derives the box-mean scale float32(fc_mean / (sum / ncells)) and its
+// density threshold, once for every block of the threshold pass; with
+// `active` the block lists the line groups whose maximum is not below
+// d* - band (the order of the list does not matter: each group writes only
+// its own mask words).
+struct SumScaleArgs {
+    ScaledThreshold* scaled;
+    float fc_mean, deltac, denom;
+    double ncells, zeta;
+    const float* bmax;
+    int* active;
+    int* nactive;
+};
+
+template <int NT>
+__device__ void finish_box_sum(const double* partial, int n, double* out, const SumScaleArgs& sa) {
+    __shared__ double part[NT / 64];
+    __shared__ DensityThreshold sth;
+    __shared__ int count;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT) acc += partial[i];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < NT / 64; ++w) r += part[w];
+        *out = r;
+        if (sa.scaled) {
+            const float scale = (float)((double)sa.fc_mean / (r / sa.ncells));
+            ScaledThreshold s;
+            s.scale = scale;
+            s.th = density_threshold(sa.deltac, sa.denom, sa.zeta, scale);
+            s.pad = 0.0f;
+            *sa.scaled = s;
+            sth = s.th;
+        }
+        count = 0;
+    }
+    if (!sa.scaled || !sa.active) return;
+    __syncthreads();
+    const float lim = sth.dstar - sth.band;
+    for (int i = threadIdx.x; i < n; i += NT)
+        if (!(sa.bmax[i] < lim)) sa.active[atomicAdd(&count, 1)] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) *sa.nactive = count;
+}
+
 // Pass 3: c2r along axis 2 for L consecutive lines, then the f_coll epilogue:
 // FP64 block partial of the box sum and the block's maximum of the smoothed
 // field (MODE kLineSum), threshold into the ionization bitmask
@@ -166,62 +217,45 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
     lines_block<N1, N2, L, MODE>(U, tw, fp, ep, blk, buf, stw, smask);
 }
 
+// Threshold pass of radius r and box-sum pass of radius r+1 in one launch
+// (nonlinear input): the first `nthr` blocks take the threshold role (with
+// the active list of radius r), the others the box-sum role.  The two read
+// different intermediates and write different buffers (mask words, and the
+// partials and maxima of radius r+1), so they need no ordering between them,
+// and the launch replaces two dependent launches per radius.
+template <int N1, int N2, int L>
+__global__ __launch_bounds__(L* N1) void k_fused_lines_pair(const float2* __restrict__ Uthr, FusedParams fpt,
+                                                           LineEpilogue ept, const float2* __restrict__ Usum,
+                                                           FusedParams fps, LineEpilogue eps,
+                                                           const float2* __restrict__ tw, int nthr) {
+    using C = LinesCore<N1, N2, L>;
+    __shared__ float2 buf[L * C::HP];
+    __shared__ float2 stw[C::D];
+    __shared__ unsigned long long smask[C::WORDS];
+    int blk = blockIdx.x;
+    if (blk < nthr) {
+        if (ept.active != nullptr) {
+            if (blk >= *ept.nactive) return;
+            blk = ept.active[blk];
+        }
+        lines_block<N1, N2, L, kLineThreshold>(Uthr, tw, fpt, ept, blk, buf, stw, smask);
+    } else {
+        lines_block<N1, N2, L, kLineSum>(Usum, tw, fps, eps, blk - nthr, buf, stw, smask);
+    }
+}
+
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
 }
 
 
-// One block of 1024 threads: strided per-thread sums, then a fixed-order tree
-// over the 16 wavefronts (deterministic).
+// One block of 1024 threads reduces the partials of a box-sum pass.
 constexpr int kSumThreads = 1024;
-
-// With `scaled` the same thread also derives the radius' box-mean scale and
-// its density threshold, once for every block of the threshold pass.
-struct SumScaleArgs {
-    ScaledThreshold* scaled;
-    float fc_mean, deltac, denom;
-    double ncells, zeta;
-    const float* bmax;  // with `active`: per-group maxima of the box-sum pass
-    int* active;        // receives the line groups that can reach the threshold
-    int* nactive;
-};
 
 __global__ __launch_bounds__(kSumThreads) void k_sum_partials_lines(const double* __restrict__ partial, int n,
                                                                     double* __restrict__ out, SumScaleArgs sa) {
-    __shared__ double part[kSumThreads / 64];
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += kSumThreads) acc += partial[i];
-    acc = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    __shared__ DensityThreshold sth;
-    __shared__ int count;
-    if (threadIdx.x == 0) {
-        double r = 0.0;
-        for (int w = 0; w < kSumThreads / 64; ++w) r += part[w];
-        *out = r;
-        if (sa.scaled) {
-            const float scale = (float)((double)sa.fc_mean / (r / sa.ncells));
-            ScaledThreshold s;
-            s.scale = scale;
-            s.th = density_threshold(sa.deltac, sa.denom, sa.zeta, scale);
-            s.pad = 0.0f;
-            *sa.scaled = s;
-            sth = s.th;
-        }
-        count = 0;
-    }
-    if (!sa.scaled || !sa.active) return;
-    __syncthreads();
-    // the line groups the threshold pass must visit: those whose maximum is
-    // not below d* - band (the order in the list does not matter: each group
-    // writes only its own mask words)
-    const float lim = sth.dstar - sth.band;
-    for (int i = threadIdx.x; i < n; i += kSumThreads)
-        if (!(sa.bmax[i] < lim)) sa.active[atomicAdd(&count, 1)] = i;
-    __syncthreads();
-    if (threadIdx.x == 0) *sa.nactive = count;
+    finish_box_sum<kSumThreads>(partial, n, out, sa);
 }
 
 // ------------------------------------------------------------------ host side
@@ -256,14 +290,24 @@ int launch_lines(int mode, const float2* U, const float2* tw, const FusedParams&
     return CM21_OK;
 }
 
+template <int N1, int N2, int L>
+int launch_lines_pair(const float2* Uthr, const FusedParams& fpt, const LineEpilogue& ept, const float2* Usum,
+                      const FusedParams& fps, const LineEpilogue& eps, const float2* tw, hipStream_t st) {
+    constexpr int D = 2 * N1 * N2;
+    const int nblk = D * D / L;
+    k_fused_lines_pair<N1, N2, L><<<2 * nblk, L * N1, 0, st>>>(Uthr, fpt, ept, Usum, fps, eps, tw, nblk);
+    CM21_LAUNCH_CHECK("k_fused_lines_pair");
+    return CM21_OK;
+}
+
 // D, columns/rows (D1, D2, W), lines (N1, N2, L)
 const FusedShape kShapes[] = {
-    {64, launch_columns<8, 8, 32>, launch_rows<8, 8, 32>, launch_lines<4, 8, 64>, 32, 64},
-    {96, launch_columns<8, 12, 32>, launch_rows<8, 12, 32>, launch_lines<4, 12, 64>, 32, 64},
-    {128, launch_columns<8, 16, 32>, launch_rows<8, 16, 32>, launch_lines<8, 8, 32>, 32, 32},
-    {160, launch_columns<16, 10, 16>, launch_rows<16, 10, 16>, launch_lines<8, 10, 32>, 16, 32},
-    {256, launch_columns<16, 16, 16>, launch_rows<16, 16, 16>, launch_lines<16, 8, 16>, 16, 16},
-    {512, launch_columns<16, 32, 8>, launch_rows<16, 32, 8>, launch_lines<16, 16, 8>, 8, 8},
+    {64, launch_columns<8, 8, 32>, launch_rows<8, 8, 32>, launch_lines<4, 8, 64>, launch_lines_pair<4, 8, 64>, 32, 64},
+    {96, launch_columns<8, 12, 32>, launch_rows<8, 12, 32>, launch_lines<4, 12, 64>, launch_lines_pair<4, 12, 64>, 32, 64},
+    {128, launch_columns<8, 16, 32>, launch_rows<8, 16, 32>, launch_lines<8, 8, 32>, launch_lines_pair<8, 8, 32>, 32, 32},
+    {160, launch_columns<16, 10, 16>, launch_rows<16, 10, 16>, launch_lines<8, 10, 32>, launch_lines_pair<8, 10, 32>, 16, 32},
+    {256, launch_columns<16, 16, 16>, launch_rows<16, 16, 16>, launch_lines<16, 8, 16>, launch_lines_pair<16, 8, 16>, 16, 16},
+    {512, launch_columns<16, 32, 8>, launch_rows<16, 32, 8>, launch_lines<16, 16, 8>, launch_lines_pair<16, 16, 8>, 8, 8},
 };
 
 const FusedShape* fused_shape(int D) {
@@ -340,6 +384,15 @@ int fused_lines(const FusedShape& sh, int mode, const float2* U, const float2* t
     return CM21_OK;
 }
 
+int fused_lines_pair(const FusedShape& sh, const float2* Uthr, const FusedParams& fpt, const LineEpilogue& ept,
+                     const float2* Usum, const FusedParams& fps, const LineEpilogue& eps, const float2* tw,
+                     hipStream_t st) {
+    ProfToken t = prof_begin(CM21_STAGE_FUSED_LINES_PAIR, st);
+    CM21_TRY(sh.pair(Uthr, fpt, ept, Usum, fps, eps, tw, st));
+    prof_end(t, st);
+    return CM21_OK;
+}
+
 int fused_mask_clear(unsigned long long* mask, size_t words, hipStream_t st) {
     k_fill_u64<<<grid_for(words), kThreads, 0, st>>>(mask, words);
     CM21_LAUNCH_CHECK("k_fill_u64");
@@ -375,6 +428,14 @@ DensityThreshold density_threshold_host(float deltac, float denom, double zeta, 
     return t;
 }
 
+int fused_sum(const double* partial, int n, double* out, hipStream_t st, ScaledThreshold* scaled, float fc_mean,
+              double ncells, float deltac, float denom, double zeta, const float* bmax, int* active, int* nactive) {
+    SumScaleArgs sa{scaled, fc_mean, deltac, denom, ncells, zeta, bmax, active, nactive};
+    k_sum_partials_lines<<<1, kSumThreads, 0, st>>>(partial, n, out, sa);
+    CM21_LAUNCH_CHECK("k_sum_partials_lines");
+    return CM21_OK;
+}
+
 LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double* sum, float fc_mean,
                            unsigned long long* mask, float* out, double* partial, double ncells, float* bmax) {
     LineEpilogue ep{};
@@ -390,14 +451,6 @@ LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double*
     ep.partial = partial;
     ep.ncells = ncells;
     return ep;
-}
-
-int fused_sum(const double* partial, int n, double* out, hipStream_t st, ScaledThreshold* scaled, float fc_mean,
-              double ncells, float deltac, float denom, double zeta, const float* bmax, int* active, int* nactive) {
-    SumScaleArgs sa{scaled, fc_mean, deltac, denom, ncells, zeta, bmax, active, nactive};
-    k_sum_partials_lines<<<1, kSumThreads, 0, st>>>(partial, n, out, sa);
-    CM21_LAUNCH_CHECK("k_sum_partials_lines");
-    return CM21_OK;
 }
 
 }  // namespace cm21

@@ -60,6 +60,8 @@ struct FusedShape {
     int (*columns)(const float2*, float2*, const float2*, const FusedParams&, int ncg, int nb, hipStream_t);
     int (*rows)(float2*, const float2*, const FusedParams&, int ncg, hipStream_t);
     int (*lines)(int mode, const float2*, const float2*, const FusedParams&, const LineEpilogue&, hipStream_t);
+    int (*pair)(const float2*, const FusedParams&, const LineEpilogue&, const float2*, const FusedParams&,
+                const LineEpilogue&, const float2*, hipStream_t);
     int W, L;
 };
 
@@ -73,6 +75,11 @@ int fused_radius(const FusedShape& sh, const float2* S, float2* T, const float2*
                  hipStream_t st);
 int fused_lines(const FusedShape& sh, int mode, const float2* U, const float2* tw, const FusedParams& fp,
                 const LineEpilogue& ep, hipStream_t st, int stage = CM21_STAGE_FUSED_LINES);
+// Threshold pass of one radius (intermediate Uthr, epilogue ept with its
+// active list) and box-sum pass of the next (Usum, eps) in one launch.
+int fused_lines_pair(const FusedShape& sh, const float2* Uthr, const FusedParams& fpt, const LineEpilogue& ept,
+                     const float2* Usum, const FusedParams& fps, const LineEpilogue& eps, const float2* tw,
+                     hipStream_t st);
 int fused_mask_clear(unsigned long long* mask, size_t words, hipStream_t st);
 // Fixed-order FP64 sum of the box-sum partials into *out; with `scaled`
 // also the box-mean scale float32(fc_mean / (sum/ncells)) and its threshold.

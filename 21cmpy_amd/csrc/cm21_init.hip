@@ -85,22 +85,11 @@ __device__ inline float table_pk(float k, const float* K, const float* Pt, int n
     return Pt[lo] + (Pt[hi] - Pt[lo]) * ((k - K[lo]) / (K[hi] - K[lo]));
 }
 
-__global__ void k_grf(float2* __restrict__ spec, int D, size_t Nc, GrfParams gp) {
-    __shared__ float sK[kMaxTable], sP[kMaxTable];
-    if (gp.ntab > 0) {
-        for (int i = threadIdx.x; i < gp.ntab; i += blockDim.x) {
-            sK[i] = gp.ktab[i];
-            sP[i] = gp.ptab[i];
-        }
-        __syncthreads();
-    }
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= Nc) return;
+// One mode: its Philox draw, amplitude and Hermitian placement.
+__device__ __forceinline__ void grf_mode(float2* __restrict__ spec, int D, int a, int b, int c, const GrfParams& gp,
+                                         const float* sK, const float* sP) {
     const int H = D / 2 + 1;
-    const int c = (int)(idx % H);
-    const size_t ab = idx / H;
-    const int b = (int)(ab % D);
-    const int a = (int)(ab / D);
+    const size_t idx = ((size_t)a * D + b) * H + c;
     if (a == 0 && b == 0 && c == 0) {
         spec[idx] = make_float2(0.0f, 0.0f);
         return;
@@ -136,13 +125,33 @@ __global__ void k_grf(float2* __restrict__ spec, int D, size_t Nc, GrfParams gp)
     }
 }
 
+// Blocks of 64 x 4 threads; block (0, y, z) covers a = z and the kGrfRows
+// rows b from y * kGrfRows, each thread walking c by 64 and b by 4, so the
+// P(k) table is loaded into LDS once per 4 * kGrfRows * H modes.
+constexpr int kGrfRows = 16;
+
+__global__ __launch_bounds__(kModeBlockX* kModeBlockY) void k_grf(float2* __restrict__ spec, int D, GrfParams gp) {
+    __shared__ float sK[kMaxTable], sP[kMaxTable];
+    if (gp.ntab > 0) {
+        const int t = threadIdx.y * kModeBlockX + threadIdx.x;
+        for (int i = t; i < gp.ntab; i += kModeBlockX * kModeBlockY) {
+            sK[i] = gp.ktab[i];
+            sP[i] = gp.ptab[i];
+        }
+        __syncthreads();
+    }
+    const int H = D / 2 + 1;
+    const int a = blockIdx.z;
+    const int b_end = min(D, (int)(blockIdx.y + 1) * kGrfRows);
+    for (int b = blockIdx.y * kGrfRows + threadIdx.y; b < b_end; b += kModeBlockY)
+        for (int c = threadIdx.x; c < H; c += kModeBlockX) grf_mode(spec, D, a, b, c, gp, sK, sP);
+}
+
 __global__ void k_subsample(float* __restrict__ small, const float* __restrict__ large, int D, int sw, float pf) {
-    const size_t n = (size_t)sw * sw * sw;
-    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int k = (int)(p % sw);
-    const int j = (int)((p / sw) % sw);
-    const int i = (int)(p / ((size_t)sw * sw));
+    const int k = blockIdx.x * kModeBlockX + threadIdx.x, j = blockIdx.y * kModeBlockY + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= sw || j >= sw) return;
+    const size_t p = ((size_t)i * sw + j) * sw + k;
     // floor(k*pixel_factor + 0.5): product in float, + 0.5 in double.
     const int lk = (int)floor((double)__fmul_rn((float)k, pf) + 0.5);
     const int lj = (int)floor((double)__fmul_rn((float)j, pf) + 0.5);
@@ -151,15 +160,13 @@ __global__ void k_subsample(float* __restrict__ small, const float* __restrict__
     small[p] = large[((size_t)li * D + lj) * D + lk];
 }
 
-__global__ void k_velocity(float2* __restrict__ out, const float2* __restrict__ in, int D, size_t Nc, float dk,
-                           int comp, int filter_type, float R) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= Nc) return;
+__global__ void k_velocity(float2* __restrict__ out, const float2* __restrict__ in, int D, float dk, int comp,
+                           int filter_type, float R) {
     const int H = D / 2 + 1;
-    const int c = (int)(idx % H);
-    const size_t ab = idx / H;
-    const int b = (int)(ab % D);
-    const int a = (int)(ab / D);
+    const int c = blockIdx.x * kModeBlockX + threadIdx.x, b = blockIdx.y * kModeBlockY + threadIdx.y;
+    const int a = blockIdx.z;
+    if (c >= H || b >= D) return;
+    const size_t idx = ((size_t)a * D + b) * H + c;
     const float k0 = __fmul_rn((float)mode_of(a, D), dk);
     const float k1 = __fmul_rn((float)mode_of(b, D), dk);
     const float k2 = __fmul_rn((float)c, dk);
@@ -186,6 +193,8 @@ __global__ void k_velocity(float2* __restrict__ out, const float2* __restrict__ 
 
 bool valid_dim(int D) { return D >= 2 && D <= 2048 && D % 2 == 0; }
 
+dim3 grf_grid(int D) { return dim3(1u, (unsigned)((D + kGrfRows - 1) / kGrfRows), (unsigned)D); }
+
 }  // namespace
 
 extern "C" int cm21_grf_init(void* spec_out, int D, double box_len, double amp, double n_index, double k_cut,
@@ -193,11 +202,10 @@ extern "C" int cm21_grf_init(void* spec_out, int D, double box_len, double amp, 
     CM21_REQUIRE(spec_out, "cm21_grf_init: NULL spectrum");
     CM21_REQUIRE(valid_dim(D), "cm21_grf_init: D=%d must be even in [2,2048]", D);
     CM21_REQUIRE(box_len > 0.0 && k_cut > 0.0 && amp >= 0.0, "cm21_grf_init: box_len, k_cut must be > 0, amp >= 0");
-    const size_t Nc = (size_t)D * D * (D / 2 + 1);
     const double vol = box_len * box_len * box_len;
     GrfParams gp{(float)(2.0 * M_PI / box_len), (float)(amp / vol), (float)n_index, (float)(1.0 / k_cut),
                  (float)(1.0 / vol), seed, box_id, nullptr, nullptr, 0};
-    k_grf<<<grid_for(Nc), kThreads, 0, (hipStream_t)stream>>>((float2*)spec_out, D, Nc, gp);
+    k_grf<<<grf_grid(D), mode_block(), 0, (hipStream_t)stream>>>((float2*)spec_out, D, gp);
     CM21_LAUNCH_CHECK("k_grf");
     return CM21_OK;
 }
@@ -210,10 +218,9 @@ extern "C" int cm21_grf_init_table(void* spec_out, int D, double box_len, const 
     CM21_REQUIRE(box_len > 0.0, "cm21_grf_init_table: box_len must be > 0");
     CM21_REQUIRE(n_table >= 2 && n_table <= kMaxTable, "cm21_grf_init_table: n_table=%d not in [2,%d]", n_table,
                  kMaxTable);
-    const size_t Nc = (size_t)D * D * (D / 2 + 1);
     GrfParams gp{(float)(2.0 * M_PI / box_len), 0.0f, 0.0f, 1.0f, (float)(1.0 / (box_len * box_len * box_len)),
                  seed, box_id, k_table, p_table, n_table};
-    k_grf<<<grid_for(Nc), kThreads, 0, (hipStream_t)stream>>>((float2*)spec_out, D, Nc, gp);
+    k_grf<<<grf_grid(D), mode_block(), 0, (hipStream_t)stream>>>((float2*)spec_out, D, gp);
     CM21_LAUNCH_CHECK("k_grf");
     return CM21_OK;
 }
@@ -225,8 +232,7 @@ extern "C" int cm21_subsample(float* small, const float* large, int D, int sw, f
     const double last = std::floor((double)((float)(sw - 1) * pixel_factor) + 0.5);
     CM21_REQUIRE(pixel_factor > 0.0f && last < (double)D, "cm21_subsample: pixel_factor=%g reads past D=%d",
                  (double)pixel_factor, D);
-    const size_t n = (size_t)sw * sw * sw;
-    k_subsample<<<grid_for(n), kThreads, 0, (hipStream_t)stream>>>(small, large, D, sw, pixel_factor);
+    k_subsample<<<mode_grid(sw, sw), mode_block(), 0, (hipStream_t)stream>>>(small, large, D, sw, pixel_factor);
     CM21_LAUNCH_CHECK("k_subsample");
     return CM21_OK;
 }
@@ -237,9 +243,8 @@ extern "C" int cm21_velocity(void* spec_out, const void* spec_in, int D, float d
     CM21_REQUIRE(valid_dim(D), "cm21_velocity: D=%d must be even in [2,2048]", D);
     CM21_REQUIRE(comp >= 0 && comp <= 2, "cm21_velocity: comp=%d not in {0,1,2}", comp);
     CM21_REQUIRE(filter_type <= 2, "cm21_velocity: filter_type=%d", filter_type);
-    const size_t Nc = (size_t)D * D * (D / 2 + 1);
-    k_velocity<<<grid_for(Nc), kThreads, 0, (hipStream_t)stream>>>((float2*)spec_out, (const float2*)spec_in, D, Nc,
-                                                                   delta_k, comp, filter_type, R);
+    k_velocity<<<mode_grid(D, D / 2 + 1), mode_block(), 0, (hipStream_t)stream>>>(
+        (float2*)spec_out, (const float2*)spec_in, D, delta_k, comp, filter_type, R);
     CM21_LAUNCH_CHECK("k_velocity");
     return CM21_OK;
 }

@@ -39,16 +39,18 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
     return v;
 }
 
-// Block-wide FP64 sum in a fixed order; thread 0 receives the total.
+// Block-wide FP64 sum in a fixed order (blocks of up to 1024 threads, a
+// multiple of 64); thread 0 receives the total.
 __device__ inline double block_sum(double v) {
-    __shared__ double part[kWaves];
+    __shared__ double part[16];
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) part[w] = v;
     __syncthreads();
     double r = 0.0;
     if (threadIdx.x == 0) {
-        for (int i = 0; i < kWaves; ++i) r += part[i];
+        const int nw = (int)(blockDim.x >> 6);
+        for (int i = 0; i < nw; ++i) r += part[i];
     }
     return r;
 }
@@ -79,13 +81,23 @@ struct SideStreams {
     hipStream_t s[kStreams] = {};
     hipEvent_t fork = nullptr;
     hipEvent_t join[kStreams] = {};
-    hipEvent_t ev[4] = {};  // free for the loop's own dependencies
+    hipEvent_t ev[6] = {};  // free for the loop's own dependencies
 };
 int side_streams(SideStreams** out);
 
 // Drop captured graphs that reference a plan about to be destroyed.
 void forget_plan_graphs(const ::cm21_plan* plan);
 void forget_batch_graphs();
+
+// Launch geometry of the per-mode kernels over a half spectrum (D, D, H):
+// blocks of 64 x 4 threads, x over the halved axis c, y over b, z over a, so
+// a thread finds its mode without any 64-bit division.
+constexpr int kModeBlockX = 64, kModeBlockY = 4;
+inline dim3 mode_grid(int D, int H) {
+    return dim3((unsigned)((H + kModeBlockX - 1) / kModeBlockX), (unsigned)((D + kModeBlockY - 1) / kModeBlockY),
+                (unsigned)D);
+}
+inline dim3 mode_block() { return dim3(kModeBlockX, kModeBlockY, 1); }
 
 // Signed mode number of FFT index i on an axis of length D; index D/2 maps
 // to +D/2 (Programs/find_bubbles.cu:22-24).

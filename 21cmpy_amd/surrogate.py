@@ -362,7 +362,7 @@ class BoxSet:
         self.target = torch.empty((n, D, D, D), dtype=torch.float32, device=dev)
         self.cond = torch.tensor([normalise_cond(s["zeta"], s["sigma_min2"]) for s in self.specs],
                                  dtype=torch.float32)
-        amp = sig.PowerLawSpectrum(D, L).amp
+        amp = sig.power_law_spectrum(D, L).amp
         for i, s in enumerate(self.specs):
             self.delta[i].copy_(g.c2r(g.grf_init(D, L, amp, s["seed"])))
         for c0 in range(0, n, chunk):

@@ -34,7 +34,9 @@ def alg_bytes(D, norm=1):
     c2r 8Nc + 4N, f_coll box sum 4N, threshold 12N.  A fused kernel carries the
     model bytes of the stages it replaces: columns = filter; lines = c2r plus
     the box sum (nonlinear input) or plus the threshold (linear input); rows
-    are the middle of the c2r and carry none of their own."""
+    are the middle of the c2r and carry none of their own; the paired launch
+    of nonlinear input carries the c2r and box sum of one radius and the
+    threshold of the previous one."""
     N = D ** 3
     Nc = D * D * (D // 2 + 1)
     return {
@@ -47,6 +49,7 @@ def alg_bytes(D, norm=1):
         "fused_rows": 0,
         "fused_lines": 8 * Nc + 4 * N + (4 * N if norm else 12 * N),
         "fused_lines_threshold": 12 * N,
+        "fused_lines_pair": 8 * Nc + 8 * N + 12 * N,
     }
 
 

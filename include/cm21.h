@@ -239,7 +239,9 @@ int cm21_power_spectrum(cm21_plan* plan, const float* field,
 #define CM21_STAGE_FUSED_LINES 8           /* the first lines pass of a radius (box sum for
                                               nonlinear input, else the threshold/paint) */
 #define CM21_STAGE_FUSED_LINES_THRESHOLD 9 /* the threshold/paint lines pass after a box sum */
-#define CM21_NSTAGES 10
+#define CM21_STAGE_FUSED_LINES_PAIR 10     /* threshold pass of radius r with the box-sum pass
+                                              of radius r+1 in one launch (nonlinear input) */
+#define CM21_NSTAGES 11
 int cm21_profile_enable(int stage_mask);
 int cm21_profile_collect(double* ms, int64_t* count, int nstages);
 

@@ -163,6 +163,28 @@ def test_fused_large_box_runs(gpu, D):
     assert 0.05 < float((a == 1).float().mean()) < 0.95
 
 
+@pytest.mark.parametrize("D", [512])
+def test_fused_large_box_nonlinear(gpu, D):
+    # nonlinear input at D=512 (128-thread line blocks): the box sums, their
+    # in-kernel reduction and the paired threshold launches against the
+    # generic path (rocFFT c2r, separate sum and threshold kernels)
+    fb = pkg("Programs.find_HII_bubbles")
+    g = pkg("gpu")
+    sig = pkg("cosmo_files.sigmas")
+    L, growth = 500.0, 0.12
+    sp = sig.PowerLawSpectrum(D, L)
+    delta = (g.c2r(g.grf_init(D, L, sp.amp, 0)) * growth).contiguous()
+    kw = dict(LE=False, quiet=True, deltac=1.686 / growth, sigma_min2=40.0, zeta=30.0, sig0=sp.sig0_discrete)
+    a = fb.conv_bubbles(delta, {"BoxSize": L, "HIIdim": D}, fused=True, **kw)
+    b = fb.conv_bubbles(delta, {"BoxSize": L, "HIIdim": D}, fused=False, **kw)
+    ion = float((b == 1).float().mean())
+    print(f"D={D} nonlinear: ionized fraction {ion:.4f}", flush=True)
+    assert 0.01 < ion < 0.99
+    assert float(((a == 1) != (b == 1)).float().mean()) < 2e-5
+    part = (a < 1) & (b < 1)
+    assert float((a[part] - b[part]).abs().max()) < 1e-4
+
+
 def test_full_size_properties(gpu):
     # D=256, L=300 (40 radii): size-independent properties of the result.
     fb = pkg("Programs.find_HII_bubbles")
