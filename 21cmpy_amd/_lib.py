@@ -157,7 +157,7 @@ SIGNATURES = {
 }
 
 STAGES = ["r2c", "filter", "c2r", "fcoll_sum", "threshold", "sphere", "fused_columns", "fused_rows",
-          "fused_lines", "fused_lines_threshold", "fused_lines_pair"]
+          "fused_lines", "fused_paint", "fused_radius"]
 
 
 def profile_enable(stages=True):

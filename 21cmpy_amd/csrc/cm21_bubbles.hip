@@ -234,41 +234,48 @@ extern "C" int cm21_update_sphere(float* ionized, const float* smoothed, int D, 
 
 // Workspace carve-up of the loop driver.
 namespace {
+// Pipelined fused loop: radius r uses intermediate r % kInter and the small
+// per-radius buffers r % kSlots; a launch touches radii r-2 .. r+2.
+constexpr int kInter = 5, kSlots = 3;
+
 struct LoopWork {
     float2* spec;
-    float2* filt;
-    float2* filt2;  // second and third intermediates of the fused loop
-    float2* filt3;  //   (radius r uses intermediate r % 3)
-    float* smoothed;
+    float2* filt;             // generic path: filtered spectrum
+    float* smoothed;          // generic path: smoothed box
+    float2* inter[kInter];    // fused path: intermediates (inter[0] is filt,
+                              // inter[1] shares smoothed's region)
     double* partial;
     double* sums;
     void* sphere;
     unsigned long long* mask;
-    double* lpart[2];  // box-sum partials, per radius parity
-    float* bmax[2];    // per line group maxima, per radius parity
-    ScaledThreshold* scaled;
-    int* active[2];    // line groups the threshold pass visits, and their number
-    int* nactive[2];
+    double* lpart[kSlots];    // box-sum partials
+    float* bmax[kSlots];      // per line group maxima
+    int* active[kSlots];      // line groups the threshold pass visits, and their number
+    int* nactive[kSlots];
+    ScaledThreshold* scaled;  // per radius
     size_t bytes;
 };
 
 LoopWork carve(void* base, int D, int n_radii, int update) {
     const size_t N = (size_t)D * D * D;
     const size_t Nc = (size_t)D * D * (D / 2 + 1);
+    const size_t Nt = (size_t)D * D * padded_row(D / 2 + 1);  // an intermediate (padded rows)
     char* p = (char*)base;
     LoopWork w{};
     size_t off = 0;
     w.spec = (float2*)(p + off);
     off = align_up(off + sizeof(float2) * Nc, 256);
-    w.filt = (float2*)(p + off);
-    off = align_up(off + sizeof(float2) * Nc, 256);
-    w.filt2 = (float2*)(p + off);
-    off = align_up(off + sizeof(float2) * Nc, 256);
-    // the generic path's smoothed box and the fused path's third
-    // intermediate share one region (8Nc >= 4N bytes)
+    w.filt = w.inter[0] = (float2*)(p + off);
+    off = align_up(off + sizeof(float2) * Nt, 256);
+    // 8Nt >= 4N bytes
     w.smoothed = (float*)(p + off);
-    w.filt3 = (float2*)(p + off);
-    off = align_up(off + std::max(sizeof(float) * N, sizeof(float2) * Nc), 256);
+    w.inter[1] = (float2*)(p + off);
+    off = align_up(off + std::max(sizeof(float) * N, sizeof(float2) * Nt), 256);
+    // the fused path (update == 0 only) needs three more intermediates
+    for (int k = 2; k < kInter && !update; ++k) {
+        w.inter[k] = (float2*)(p + off);
+        off = align_up(off + sizeof(float2) * Nt, 256);
+    }
     w.partial = (double*)(p + off);
     off = align_up(off + cm21_reduce_workspace_bytes(), 256);
     w.sums = (double*)(p + off);
@@ -277,7 +284,7 @@ LoopWork carve(void* base, int D, int n_radii, int update) {
     if (update) off = align_up(off + cm21_sphere_workspace_bytes(D), 256);
     w.mask = (unsigned long long*)(p + off);
     off = align_up(off + sizeof(unsigned long long) * ((N + 63) / 64), 256);
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < kSlots; ++k) {
         w.lpart[k] = (double*)(p + off);
         off = align_up(off + sizeof(double) * (fused_partials(D) + 1), 256);
         w.bmax[k] = (float*)(p + off);
@@ -362,104 +369,93 @@ int run_loop(cm21_plan* plan, const float* delta, float* ionized, const float* r
     if (prm->fused && sh && prm->update == 0 && (prm->filter_type == 1 || prm->filter_type == 2)) {
         const float2* tw = plan_twiddles(plan);
         CM21_TRY(fused_mask_clear(w.mask, (N + 63) / 64, st));
-        // Two streams, joined by events (captured into the same graph when
-        // the caller's stream is capturing).  The columns and rows passes of
-        // a radius depend only on the pristine spectrum and a free
-        // intermediate, so they run ahead on side stream ss into one of
-        // three intermediates (radius r uses T[r % 3]); the lines passes stay
-        // on the caller's stream st.
-        //
-        // Nonlinear input: the box-sum pass of radius r (c2r along the last
-        // axis, FP64 partials and per-group maxima) and the threshold pass of
-        // radius r-1 (which recomputes the last-axis transform, in the listed
-        // groups only, rather than storing and re-reading the smoothed field)
-        // form one launch; a one-block reduction then derives the box-mean
-        // scale and density threshold of radius r and lists the line groups
-        // whose maximum can reach it.  So st runs two launches per radius,
-        // and T[(r-1) % 3] is free once the first is done.
-        // Linear input: one threshold pass per radius, threshold from the
-        // host (scale 1); T[r % 3] is free after it.
-        SideStreams* ov = nullptr;
-        CM21_TRY(side_streams(&ov));
-        std::lock_guard<std::mutex> lk(ov->mu);
-        hipStream_t ss = ov->s[0];
-        hipEvent_t* rows_done = ov->ev;       // [3]
-        hipEvent_t* lines_done = ov->ev + 3;  // [3]
-        float2* T[3] = {w.filt, w.filt2, w.filt3};
+        // One launch per radius on the caller's stream, each with up to five
+        // independent roles over neighbouring radii (k_fused_radius): launch
+        // j holds the columns pass of radius j+2, the rows pass of radius j+1,
+        // the box-sum pass of radius j, the box-sum reduction of radius j-1
+        // and the threshold pass of radius j-2 (nonlinear input), or the
+        // threshold pass of radius j (linear input, threshold from the host).
+        // In-order launches carry every dependency (rows after columns, sum
+        // after rows, reduction after sum, threshold after reduction), so the
+        // loop needs no events, and the pruned transforms of later radii
+        // share the chip with the box sums instead of waiting for them.
+        // The threshold pass recomputes the last-axis transform in the line
+        // groups whose maximum can reach the threshold rather than storing and
+        // re-reading the smoothed field; the final paint is one more launch.
+        const int nblk = (int)fused_partials(D);
+        const bool nl = !prm->linear_input;
         auto params = [&](int r) {
             return fused_params(D, dk, radii[r], prm->filter_type, prm->ref_fallthrough, scale);
         };
-        auto issue_radius = [&](int r) -> int {
-            const int k = r % 3;
-            if (r >= 3) CM21_HIP_CHECK(hipStreamWaitEvent(ss, lines_done[k], 0));
-            CM21_TRY(fused_radius(*sh, w.spec, T[k], tw, params(r), ss));
-            CM21_HIP_CHECK(hipEventRecord(rows_done[k], ss));
-            return CM21_OK;
-        };
-        // box-sum epilogue of radius r (nonlinear input)
-        auto sum_epilogue = [&](int r) {
-            const int q = r & 1;
-            return line_epilogue(prm->deltac, denoms[r], prm->zeta, nullptr, prm->fc_mean, w.mask, ionized,
-                                 w.lpart[q], (double)N, w.bmax[q]);
-        };
-        // T[k] is free: issue the columns and rows of the radius that uses it next
-        auto release = [&](int r) -> int {
-            CM21_HIP_CHECK(hipEventRecord(lines_done[r % 3], st));
-            if (r + 3 < n_radii) CM21_TRY(issue_radius(r + 3));
-            return CM21_OK;
-        };
-        // threshold (or, at the last radius, final paint) epilogue of radius r
         auto threshold_epilogue = [&](int r) {
-            const int q = r & 1;
-            LineEpilogue e;
-            if (prm->linear_input) {
-                e = line_epilogue(prm->deltac, denoms[r], prm->zeta, nullptr, prm->fc_mean, w.mask, ionized,
-                                  w.lpart[q], (double)N, nullptr);
-                e.th = density_threshold_host(prm->deltac, denoms[r], prm->zeta, 1.0f);
-            } else {
-                e = line_epilogue(prm->deltac, denoms[r], prm->zeta, w.sums + r, prm->fc_mean, w.mask, ionized,
-                                  w.lpart[q], (double)N, w.bmax[q]);
+            const int q = r % kSlots;
+            LineEpilogue e = line_epilogue(prm->deltac, denoms[r], prm->zeta, nl ? w.sums + r : nullptr,
+                                           prm->fc_mean, w.mask, ionized, w.lpart[q], (double)N,
+                                           nl ? w.bmax[q] : nullptr);
+            if (nl) {
                 e.scaled = w.scaled + r;
                 if (r < n_radii - 1) {
                     e.active = w.active[q];
                     e.nactive = w.nactive[q];
                 }
+            } else {
+                e.th = density_threshold_host(prm->deltac, denoms[r], prm->zeta, 1.0f);
             }
             return e;
         };
-        CM21_HIP_CHECK(hipEventRecord(ov->fork, st));
-        CM21_HIP_CHECK(hipStreamWaitEvent(ss, ov->fork, 0));
-        for (int r = 0; r < n_radii && r < 3; ++r) CM21_TRY(issue_radius(r));
-        for (int r = 0; r < n_radii; ++r) {
-            const int k = r % 3;
-            CM21_HIP_CHECK(hipStreamWaitEvent(st, rows_done[k], 0));
-            if (prm->linear_input) {
-                CM21_TRY(fused_lines(*sh, r == n_radii - 1 ? 2 : 1, T[k], tw, params(r), threshold_epilogue(r), st));
-                CM21_TRY(release(r));
-                continue;
+        const int last = n_radii - 1;
+        // launches j = -2 .. j_end-1: the last one holds the reduction of the
+        // last radius and the threshold pass of the one before (nonlinear), or
+        // the threshold pass of the radius before the last (linear)
+        const int j_end = nl ? n_radii + 1 : n_radii - 1;
+        for (int j = -2; j < j_end; ++j) {
+            RadiusRoles a{};
+            if (j + 2 <= last) {
+                const int r = j + 2;
+                a.S = w.spec;
+                a.Tcols = w.inter[r % kInter];
+                a.fp_cols = params(r);
+                int nb = 0;
+                column_groups(*sh, a.fp_cols, &a.ncg_cols, &nb);
+                a.n_cols = a.ncg_cols * nb;
             }
-            if (r == 0) {
-                CM21_TRY(fused_lines(*sh, 0, T[k], tw, params(r), sum_epilogue(r), st));
-            } else {
-                CM21_TRY(fused_lines_pair(*sh, T[(r - 1) % 3], params(r - 1), threshold_epilogue(r - 1), T[k],
-                                          params(r), sum_epilogue(r), tw, st));
-                CM21_TRY(release(r - 1));
+            a.n_mem = a.n_cols;
+            if (j + 1 >= 0 && j + 1 <= last) {
+                const int r = j + 1;
+                a.Trows = w.inter[r % kInter];
+                a.fp_rows = params(r);
+                int nb = 0;
+                column_groups(*sh, a.fp_rows, &a.ncg_rows, &nb);
+                a.n_mem += a.ncg_rows * D;
             }
-            const int q = r & 1;
-            const bool last = r == n_radii - 1;
-            CM21_TRY(fused_sum(w.lpart[q], (int)fused_partials(D), w.sums + r, st, w.scaled + r, prm->fc_mean,
-                               (double)N, prm->deltac, denoms[r], prm->zeta, w.bmax[q], last ? nullptr : w.active[q],
-                               last ? nullptr : w.nactive[q]));
+            if (nl && j - 1 >= 0 && j - 1 <= last) {
+                const int r = j - 1, q = r % kSlots;
+                a.partial = w.lpart[q];
+                a.npart = nblk;
+                a.total = w.sums + r;
+                a.sa = SumScaleArgs{w.scaled + r, prm->fc_mean, prm->deltac, denoms[r], (double)N, prm->zeta,
+                                    w.bmax[q], r < last ? w.active[q] : nullptr, r < last ? w.nactive[q] : nullptr};
+                a.n_red = 1;
+            }
+            const int rt = nl ? j - 2 : j;  // the threshold pass (the last radius is painted below)
+            if (rt >= 0 && rt < last) {
+                a.Uthr = w.inter[rt % kInter];
+                a.fp_thr = params(rt);
+                a.ep_thr = threshold_epilogue(rt);
+                a.n_thr = nblk;
+            }
+            if (nl && j >= 0 && j <= last) {
+                const int r = j, q = r % kSlots;
+                a.Usum = w.inter[r % kInter];
+                a.fp_sum = params(r);
+                a.ep_sum = line_epilogue(prm->deltac, denoms[r], prm->zeta, nullptr, prm->fc_mean, w.mask, ionized,
+                                         w.lpart[q], (double)N, w.bmax[q]);
+                a.n_sum = nblk;
+            }
+            CM21_TRY(fused_radius_launch(*sh, a, tw, st));
         }
-        if (!prm->linear_input) {
-            const int r = n_radii - 1;
-            CM21_TRY(fused_lines(*sh, 2, T[r % 3], tw, params(r), threshold_epilogue(r), st,
-                                 CM21_STAGE_FUSED_LINES_THRESHOLD));
-        }
-        // join the side stream back (a capture requires every forked stream
-        // to be joined by the origin stream)
-        CM21_HIP_CHECK(hipEventRecord(ov->join[0], ss));
-        CM21_HIP_CHECK(hipStreamWaitEvent(st, ov->join[0], 0));
+        CM21_TRY(fused_lines(*sh, 2, w.inter[last % kInter], tw, params(last), threshold_epilogue(last), st,
+                             CM21_STAGE_FUSED_LINES_THRESHOLD));
         return CM21_OK;
     }
     k_fill<<<grid_for(N), kThreads, 0, st>>>(ionized, N, 0.0f);

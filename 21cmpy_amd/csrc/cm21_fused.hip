@@ -30,24 +30,6 @@
 
 namespace cm21 {
 
-// Pass 1: window + inverse DFT along axis 0; grid (column groups, b slots).
-template <int D1, int D2, int W>
-__global__ __launch_bounds__(W* D1) void k_fused_columns(const float2* __restrict__ S, float2* __restrict__ T,
-                                                        const float2* __restrict__ tw, FusedParams fp) {
-    constexpr int D = D1 * D2;
-    __shared__ float2 lds[W * (D + 1)];
-    columns_body<D1, D2, W>(S, T, D / 2 + 1, tw, fp, fp.scale, blockIdx.y, blockIdx.x, lds);
-}
-
-// Pass 2: inverse DFT along axis 1, in place; grid (column groups, p).
-template <int D1, int D2, int W>
-__global__ __launch_bounds__(W* D1) void k_fused_rows(float2* __restrict__ T, const float2* __restrict__ tw,
-                                                     FusedParams fp) {
-    constexpr int D = D1 * D2;
-    __shared__ float2 lds[W * (D + 1)];
-    rows_body<D1, D2, W>(T, D / 2 + 1, tw, fp, blockIdx.y, blockIdx.x, lds);
-}
-
 // Box-sum reduction of one radius by the NT threads of one block: strided
 // per-thread FP64 sums, the wavefront tree, then the waves in order (the order
 // is fixed, so the total is bitwise reproducible).  With `scaled` thread 0
@@ -56,15 +38,6 @@ __global__ __launch_bounds__(W* D1) void k_fused_rows(float2* __restrict__ T, co
 // `active` the block lists the line groups whose maximum is not below
 // d* - band (the order of the list does not matter: each group writes only
 // its own mask words).
-struct SumScaleArgs {
-    ScaledThreshold* scaled;
-    float fc_mean, deltac, denom;
-    double ncells, zeta;
-    const float* bmax;
-    int* active;
-    int* nactive;
-};
-
 template <int NT>
 __device__ void finish_box_sum(const double* partial, int n, double* out, const SumScaleArgs& sa) {
     __shared__ double part[NT / 64];
@@ -143,7 +116,7 @@ __device__ __forceinline__ void lines_block(const float2* __restrict__ U, const 
     }
     // all global loads in flight before the first LDS store
     float2 tmp[C::LOADS];
-    C::load(U + line0 * H, H, fp.cmax + 1, tmp);
+    C::load(U + line0 * C::HO, C::HO, fp.cmax + 1, tmp);
     for (int q = t; q < D; q += NT) stw[q] = tw[q];
     C::store(tmp, buf);
     if (MODE != kLineSum && t < WORDS) smask[t] = mword;
@@ -217,30 +190,69 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
     lines_block<N1, N2, L, MODE>(U, tw, fp, ep, blk, buf, stw, smask);
 }
 
-// Threshold pass of radius r and box-sum pass of radius r+1 in one launch
-// (nonlinear input): the first `nthr` blocks take the threshold role (with
-// the active list of radius r), the others the box-sum role.  The two read
-// different intermediates and write different buffers (mask words, and the
-// partials and maxima of radius r+1), so they need no ordering between them,
-// and the launch replaces two dependent launches per radius.
-template <int N1, int N2, int L>
-__global__ __launch_bounds__(L* N1) void k_fused_lines_pair(const float2* __restrict__ Uthr, FusedParams fpt,
-                                                           LineEpilogue ept, const float2* __restrict__ Usum,
-                                                           FusedParams fps, LineEpilogue eps,
-                                                           const float2* __restrict__ tw, int nthr) {
+// One launch of the pipelined radius loop.  Its blocks take one of five
+// roles: the columns pass of radius r+2 and the rows pass of radius r+1 (into
+// intermediates no other role of the launch touches), the one-block box-sum
+// reduction of radius r-1 (scale, threshold and active list), the threshold
+// pass of radius r-2 and the box-sum pass of radius r.  Every role reads only
+// what earlier launches wrote and writes buffers no other role of the launch
+// reads or writes, so the roles need no ordering among themselves; in-order
+// launches on one stream order the radii.
+// Block order: the reduction, then the threshold blocks (block i takes the
+// i-th listed line group; those past the end of the list return at once),
+// then the columns/rows blocks interleaved one for one with the box-sum
+// blocks, so that the memory-bound transposes and the VALU-bound box sums
+// share every CU.
+// Eight waves per SIMD: the box-sum role (about 52 VGPRs) keeps full
+// occupancy; the columns/rows roles spill a few registers to scratch instead,
+// which costs them less than half the occupancy would cost the box sums.
+template <int D1, int D2, int W, int N1, int N2, int L>
+__global__ __launch_bounds__(W* D1, 8) void k_fused_radius(RadiusRoles a, const float2* __restrict__ tw) {
     using C = LinesCore<N1, N2, L>;
-    __shared__ float2 buf[L * C::HP];
-    __shared__ float2 stw[C::D];
-    __shared__ unsigned long long smask[C::WORDS];
-    int blk = blockIdx.x;
-    if (blk < nthr) {
-        if (ept.active != nullptr) {
-            if (blk >= *ept.nactive) return;
-            blk = ept.active[blk];
+    constexpr int D = D1 * D2, NT = W * D1;
+    static_assert(C::D == D && C::NT == NT, "columns/rows and lines tiles must share D and the block size");
+    // the columns/rows transposes run in two rounds when their one-round
+    // tile would need more LDS than the lines pass
+    constexpr int kLdsLines = L * C::HP + D + C::WORDS;
+    constexpr int kRounds = W * (D + 1) > kLdsLines + kLdsLines / 4 ? 2 : 1;
+    constexpr int kLdsCR = W * (D / kRounds + 1);
+    __shared__ float2 lds[kLdsCR > kLdsLines ? kLdsCR : kLdsLines];
+    float2* buf = lds;
+    float2* stw = lds + L * C::HP;
+    unsigned long long* smask = reinterpret_cast<unsigned long long*>(lds + L * C::HP + D);
+    int b = blockIdx.x;
+    if (b < a.n_red) {
+        finish_box_sum<NT>(a.partial, a.npart, a.total, a.sa);
+        return;
+    }
+    b -= a.n_red;
+    if (b < a.n_thr) {
+        if (a.ep_thr.active != nullptr) {
+            if (b >= *a.ep_thr.nactive) return;
+            b = a.ep_thr.active[b];
         }
-        lines_block<N1, N2, L, kLineThreshold>(Uthr, tw, fpt, ept, blk, buf, stw, smask);
+        lines_block<N1, N2, L, kLineThreshold>(a.Uthr, tw, a.fp_thr, a.ep_thr, b, buf, stw, smask);
+        return;
+    }
+    b -= a.n_thr;
+    const int pairs = a.n_mem < a.n_sum ? a.n_mem : a.n_sum;
+    bool mem;
+    int i;
+    if (b < 2 * pairs) {
+        mem = (b & 1) == 0;
+        i = b >> 1;
     } else {
-        lines_block<N1, N2, L, kLineSum>(Usum, tw, fps, eps, blk - nthr, buf, stw, smask);
+        mem = a.n_mem > a.n_sum;
+        i = pairs + (b - 2 * pairs);
+    }
+    if (!mem) {
+        lines_block<N1, N2, L, kLineSum>(a.Usum, tw, a.fp_sum, a.ep_sum, i, buf, stw, smask);
+    } else if (i < a.n_cols) {
+        columns_body<D1, D2, W, kRounds>(a.S, a.Tcols, C::HO, tw, a.fp_cols, a.fp_cols.scale, i / a.ncg_cols,
+                                i % a.ncg_cols, lds);
+    } else {
+        i -= a.n_cols;
+        rows_body<D1, D2, W, kRounds>(a.Trows, C::HO, tw, a.fp_rows, i / a.ncg_rows, i % a.ncg_rows, lds);
     }
 }
 
@@ -250,30 +262,7 @@ __global__ void k_fill_u64(unsigned long long* __restrict__ p, size_t n) {
 }
 
 
-// One block of 1024 threads reduces the partials of a box-sum pass.
-constexpr int kSumThreads = 1024;
-
-__global__ __launch_bounds__(kSumThreads) void k_sum_partials_lines(const double* __restrict__ partial, int n,
-                                                                    double* __restrict__ out, SumScaleArgs sa) {
-    finish_box_sum<kSumThreads>(partial, n, out, sa);
-}
-
 // ------------------------------------------------------------------ host side
-
-template <int D1, int D2, int W>
-int launch_columns(const float2* S, float2* T, const float2* tw, const FusedParams& fp, int ncg, int nb,
-                   hipStream_t st) {
-    k_fused_columns<D1, D2, W><<<dim3(ncg, nb), W * D1, 0, st>>>(S, T, tw, fp);
-    CM21_LAUNCH_CHECK("k_fused_columns");
-    return CM21_OK;
-}
-
-template <int D1, int D2, int W>
-int launch_rows(float2* T, const float2* tw, const FusedParams& fp, int ncg, hipStream_t st) {
-    k_fused_rows<D1, D2, W><<<dim3(ncg, D1 * D2), W * D1, 0, st>>>(T, tw, fp);
-    CM21_LAUNCH_CHECK("k_fused_rows");
-    return CM21_OK;
-}
 
 template <int N1, int N2, int L>
 int launch_lines(int mode, const float2* U, const float2* tw, const FusedParams& fp, const LineEpilogue& ep,
@@ -290,24 +279,23 @@ int launch_lines(int mode, const float2* U, const float2* tw, const FusedParams&
     return CM21_OK;
 }
 
-template <int N1, int N2, int L>
-int launch_lines_pair(const float2* Uthr, const FusedParams& fpt, const LineEpilogue& ept, const float2* Usum,
-                      const FusedParams& fps, const LineEpilogue& eps, const float2* tw, hipStream_t st) {
-    constexpr int D = 2 * N1 * N2;
-    const int nblk = D * D / L;
-    k_fused_lines_pair<N1, N2, L><<<2 * nblk, L * N1, 0, st>>>(Uthr, fpt, ept, Usum, fps, eps, tw, nblk);
-    CM21_LAUNCH_CHECK("k_fused_lines_pair");
+template <int D1, int D2, int W, int N1, int N2, int L>
+int launch_radius(const RadiusRoles& a, const float2* tw, hipStream_t st) {
+    const int blocks = a.n_red + a.n_thr + a.n_mem + a.n_sum;
+    if (blocks <= 0) return CM21_OK;
+    k_fused_radius<D1, D2, W, N1, N2, L><<<blocks, W * D1, 0, st>>>(a, tw);
+    CM21_LAUNCH_CHECK("k_fused_radius");
     return CM21_OK;
 }
 
-// D, columns/rows (D1, D2, W), lines (N1, N2, L)
+// D, lines (N1, N2, L), radius (columns/rows D1, D2, W; lines N1, N2, L)
 const FusedShape kShapes[] = {
-    {64, launch_columns<8, 8, 32>, launch_rows<8, 8, 32>, launch_lines<4, 8, 64>, launch_lines_pair<4, 8, 64>, 32, 64},
-    {96, launch_columns<8, 12, 32>, launch_rows<8, 12, 32>, launch_lines<4, 12, 64>, launch_lines_pair<4, 12, 64>, 32, 64},
-    {128, launch_columns<8, 16, 32>, launch_rows<8, 16, 32>, launch_lines<8, 8, 32>, launch_lines_pair<8, 8, 32>, 32, 32},
-    {160, launch_columns<16, 10, 16>, launch_rows<16, 10, 16>, launch_lines<8, 10, 32>, launch_lines_pair<8, 10, 32>, 16, 32},
-    {256, launch_columns<16, 16, 16>, launch_rows<16, 16, 16>, launch_lines<16, 8, 16>, launch_lines_pair<16, 8, 16>, 16, 16},
-    {512, launch_columns<16, 32, 8>, launch_rows<16, 32, 8>, launch_lines<16, 16, 8>, launch_lines_pair<16, 16, 8>, 8, 8},
+    {64, launch_lines<4, 8, 64>, launch_radius<8, 8, 32, 4, 8, 64>, 32, 64},
+    {96, launch_lines<4, 12, 64>, launch_radius<8, 12, 32, 4, 12, 64>, 32, 64},
+    {128, launch_lines<8, 8, 32>, launch_radius<8, 16, 32, 8, 8, 32>, 32, 32},
+    {160, launch_lines<8, 10, 32>, launch_radius<16, 10, 16, 8, 10, 32>, 16, 32},
+    {256, launch_lines<16, 8, 16>, launch_radius<16, 16, 16, 16, 8, 16>, 16, 16},
+    {512, launch_lines<16, 16, 8>, launch_radius<16, 32, 8, 16, 16, 8>, 8, 8},
 };
 
 const FusedShape* fused_shape(int D) {
@@ -358,22 +346,13 @@ FusedParams fused_params(int D, float dk, float R, int filter_type, int ref_fall
     return fp;
 }
 
-int fused_radius(const FusedShape& sh, const float2* S, float2* T, const float2* tw, const FusedParams& fp,
-                 hipStream_t st) {
+void column_groups(const FusedShape& sh, const FusedParams& fp, int* ncg, int* nb) {
     const int D = sh.D, H = D / 2 + 1;
     const int bpos = fp.bm < D / 2 ? fp.bm : D / 2;
     const int bneg = fp.bm < D / 2 - 1 ? fp.bm : D / 2 - 1;
-    const int nb = bpos + 1 + bneg;
-    int ncg = fp.cmax / sh.W + 1;
+    *nb = bpos + 1 + bneg;
     const int ncg_all = (H + sh.W - 1) / sh.W;
-    if (ncg > ncg_all) ncg = ncg_all;
-    ProfToken t = prof_begin(CM21_STAGE_FUSED_COLUMNS, st);
-    CM21_TRY(sh.columns(S, T, tw, fp, ncg, nb, st));
-    prof_end(t, st);
-    t = prof_begin(CM21_STAGE_FUSED_ROWS, st);
-    CM21_TRY(sh.rows(T, tw, fp, ncg, st));
-    prof_end(t, st);
-    return CM21_OK;
+    *ncg = std::min(fp.cmax / sh.W + 1, ncg_all);
 }
 
 int fused_lines(const FusedShape& sh, int mode, const float2* U, const float2* tw, const FusedParams& fp,
@@ -384,11 +363,9 @@ int fused_lines(const FusedShape& sh, int mode, const float2* U, const float2* t
     return CM21_OK;
 }
 
-int fused_lines_pair(const FusedShape& sh, const float2* Uthr, const FusedParams& fpt, const LineEpilogue& ept,
-                     const float2* Usum, const FusedParams& fps, const LineEpilogue& eps, const float2* tw,
-                     hipStream_t st) {
-    ProfToken t = prof_begin(CM21_STAGE_FUSED_LINES_PAIR, st);
-    CM21_TRY(sh.pair(Uthr, fpt, ept, Usum, fps, eps, tw, st));
+int fused_radius_launch(const FusedShape& sh, const RadiusRoles& a, const float2* tw, hipStream_t st) {
+    ProfToken t = prof_begin(CM21_STAGE_FUSED_RADIUS, st);
+    CM21_TRY(sh.radius(a, tw, st));
     prof_end(t, st);
     return CM21_OK;
 }
@@ -426,14 +403,6 @@ DensityThreshold density_threshold_host(float deltac, float denom, double zeta, 
     t.dstar = (float)ds;
     t.band = (float)(1e-4 * (1.0 + std::fabs((double)deltac) + (double)denom + std::fabs(ds)));
     return t;
-}
-
-int fused_sum(const double* partial, int n, double* out, hipStream_t st, ScaledThreshold* scaled, float fc_mean,
-              double ncells, float deltac, float denom, double zeta, const float* bmax, int* active, int* nactive) {
-    SumScaleArgs sa{scaled, fc_mean, deltac, denom, ncells, zeta, bmax, active, nactive};
-    k_sum_partials_lines<<<1, kSumThreads, 0, st>>>(partial, n, out, sa);
-    CM21_LAUNCH_CHECK("k_sum_partials_lines");
-    return CM21_OK;
 }
 
 LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double* sum, float fc_mean,

@@ -55,14 +55,48 @@ struct LineEpilogue {
     const int* nactive;        //   and their number
 };
 
+// Arguments of the box-sum reduction of one radius: the FP64 total, and for
+// nonlinear input the box-mean scale float32(fc_mean / (sum / ncells)), its
+// density threshold and the list of line groups whose maximum can reach it.
+struct SumScaleArgs {
+    ScaledThreshold* scaled;
+    float fc_mean, deltac, denom;
+    double ncells, zeta;
+    const float* bmax;
+    int* active;
+    int* nactive;
+};
+
+// The roles of one launch of the pipelined radius loop (k_fused_radius) and
+// their block counts; a count of 0 is a role the launch does not have.
+struct RadiusRoles {
+    int n_red;               // box-sum reduction: 0 or 1 block
+    int n_thr;               // threshold blocks (one line group each)
+    int n_cols, n_mem;       // columns blocks; columns + rows blocks
+    int n_sum;               // box-sum blocks (one line group each)
+    int ncg_cols, ncg_rows;  // column groups of the columns and rows roles
+    const float2* S;         // columns: pristine spectrum -> Tcols
+    float2* Tcols;
+    FusedParams fp_cols;
+    float2* Trows;           // rows: in place
+    FusedParams fp_rows;
+    const double* partial;   // reduction: npart box-sum partials -> total
+    int npart;
+    double* total;
+    SumScaleArgs sa;
+    const float2* Uthr;      // threshold pass
+    FusedParams fp_thr;
+    LineEpilogue ep_thr;
+    const float2* Usum;      // box-sum pass
+    FusedParams fp_sum;
+    LineEpilogue ep_sum;
+};
+
 struct FusedShape {
     int D;
-    int (*columns)(const float2*, float2*, const float2*, const FusedParams&, int ncg, int nb, hipStream_t);
-    int (*rows)(float2*, const float2*, const FusedParams&, int ncg, hipStream_t);
     int (*lines)(int mode, const float2*, const float2*, const FusedParams&, const LineEpilogue&, hipStream_t);
-    int (*pair)(const float2*, const FusedParams&, const LineEpilogue&, const float2*, const FusedParams&,
-                const LineEpilogue&, const float2*, hipStream_t);
-    int W, L;
+    int (*radius)(const RadiusRoles&, const float2* tw, hipStream_t);
+    int W, L;  // columns per column group, lines per line group
 };
 
 const FusedShape* fused_shape(int D);
@@ -71,22 +105,12 @@ FusedParams fused_params(int D, float dk, float R, int filter_type, int ref_fall
 LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double* sum, float fc_mean,
                            unsigned long long* mask, float* out, double* partial, double ncells,
                            float* bmax = nullptr);
-int fused_radius(const FusedShape& sh, const float2* S, float2* T, const float2* tw, const FusedParams& fp,
-                 hipStream_t st);
+// Column groups and surviving b rows of the columns role at one radius
+// (the rows role uses the same column groups over all D rows p).
+void column_groups(const FusedShape& sh, const FusedParams& fp, int* ncg, int* nb);
+int fused_radius_launch(const FusedShape& sh, const RadiusRoles& a, const float2* tw, hipStream_t st);
 int fused_lines(const FusedShape& sh, int mode, const float2* U, const float2* tw, const FusedParams& fp,
                 const LineEpilogue& ep, hipStream_t st, int stage = CM21_STAGE_FUSED_LINES);
-// Threshold pass of one radius (intermediate Uthr, epilogue ept with its
-// active list) and box-sum pass of the next (Usum, eps) in one launch.
-int fused_lines_pair(const FusedShape& sh, const float2* Uthr, const FusedParams& fpt, const LineEpilogue& ept,
-                     const float2* Usum, const FusedParams& fps, const LineEpilogue& eps, const float2* tw,
-                     hipStream_t st);
 int fused_mask_clear(unsigned long long* mask, size_t words, hipStream_t st);
-// Fixed-order FP64 sum of the box-sum partials into *out; with `scaled`
-// also the box-mean scale float32(fc_mean / (sum/ncells)) and its threshold.
-// With bmax, active and nactive it also lists the line groups whose maximum
-// can reach the threshold (for the threshold pass).
-int fused_sum(const double* partial, int n, double* out, hipStream_t st, ScaledThreshold* scaled = nullptr,
-              float fc_mean = 0.0f, double ncells = 1.0, float deltac = 0.0f, float denom = 1.0f,
-              double zeta = 1.0, const float* bmax = nullptr, int* active = nullptr, int* nactive = nullptr);
 
 }  // namespace cm21

@@ -41,14 +41,49 @@ __device__ inline float2 fused_window(float2 x, int a, int b, int c, int m2, int
     return make_float2(__fmul_rn(x.x, scale), __fmul_rn(x.y, scale));
 }
 
+// Second half of the four-step DFT of W columns of length D = D1*D2: thread
+// (j, n1) holds the D2-point DFT outputs v[k2] of column j; they are
+// twiddled by w_D^(n1 k2), transposed through LDS, and the D1-point DFTs over
+// n1 run per (column, k2) item, which hands element k2 + D2*k1 of column jj to
+// store(jj, k2, k1, value).  With ROUNDS = 2 the transpose runs in two halves
+// of k2 through a tile half the size (W*(D/2+1) float2 instead of W*(D+1)),
+// so that a launch sharing its LDS with the lines pass keeps its occupancy.
+template <int D1, int D2, int W, int ROUNDS, class Store>
+__device__ __forceinline__ void transpose_finish(float2 (&v)[D2], const float2* __restrict__ tw, float2* lds,
+                                                 Store store) {
+    constexpr int NT = W * D1, K2R = D2 / ROUNDS, LD = K2R * D1 + 1;
+    static_assert(D2 % ROUNDS == 0, "rounds must split k2 evenly");
+    const int t = threadIdx.x;
+    const int j = t % W, n1 = t / W;
+#pragma unroll
+    for (int h = 0; h < ROUNDS; ++h) {
+        if (h) __syncthreads();  // the previous round's reads are done
+#pragma unroll
+        for (int kk = 0; kk < K2R; ++kk) {
+            const int k2 = h * K2R + kk;
+            lds[j * LD + kk * D1 + n1] = (n1 * k2 == 0) ? v[k2] : cmul(v[k2], tw[n1 * k2]);
+        }
+        __syncthreads();
+        for (int it = t; it < W * K2R; it += NT) {
+            const int jj = it % W, kk = it / W;
+            float2 u[D1];
+#pragma unroll
+            for (int n = 0; n < D1; ++n) u[n] = lds[jj * LD + kk * D1 + n];
+            dft<D1>(u);
+#pragma unroll
+            for (int k1 = 0; k1 < D1; ++k1) store(jj, h * K2R + kk, k1, u[k1]);
+        }
+    }
+}
+
 // Columns pass of one block: window + inverse DFT along axis 0 for the W
 // columns (b, c0..c0+W-1), b given by its slot among the surviving rows
-// (0..bpos positive, then the negative modes).  lds: W*(D+1) float2.
-template <int D1, int D2, int W>
+// (0..bpos positive, then the negative modes).  lds: W*(D/ROUNDS+1) float2.
+template <int D1, int D2, int W, int ROUNDS = 1>
 __device__ __forceinline__ void columns_body(const float2* __restrict__ S, float2* __restrict__ T, int ho,
                                              const float2* __restrict__ tw, const FusedParams& fp, float scale,
                                              int slot, int cgroup, float2* lds) {
-    constexpr int D = D1 * D2, H = D / 2 + 1, NT = W * D1, LD = D + 1;
+    constexpr int D = D1 * D2, H = D / 2 + 1;
     const int t = threadIdx.x;
     const int j = t % W, n1 = t / W;
     const int bpos = fp.bm < D / 2 ? fp.bm : D / 2;
@@ -69,29 +104,18 @@ __device__ __forceinline__ void columns_body(const float2* __restrict__ S, float
         v[n2] = x;
     }
     dft<D2>(v);
-#pragma unroll
-    for (int k2 = 0; k2 < D2; ++k2) lds[j * LD + k2 * D1 + n1] = (n1 * k2 == 0) ? v[k2] : cmul(v[k2], tw[n1 * k2]);
-    __syncthreads();
-    for (int it = t; it < W * D2; it += NT) {
-        const int jj = it % W, k2 = it / W;
+    transpose_finish<D1, D2, W, ROUNDS>(v, tw, lds, [&](int jj, int k2, int k1, float2 u) {
         const int cc = c0 + jj;
-        float2 u[D1];
-#pragma unroll
-        for (int n = 0; n < D1; ++n) u[n] = lds[jj * LD + k2 * D1 + n];
-        dft<D1>(u);
-        if (cc < H && bs * bs + cc * cc <= fp.m2max) {
-#pragma unroll
-            for (int k1 = 0; k1 < D1; ++k1) T[((size_t)(k2 + D2 * k1) * D + b) * ho + cc] = u[k1];
-        }
-    }
+        if (cc < H && bs * bs + cc * cc <= fp.m2max) T[((size_t)(k2 + D2 * k1) * D + b) * ho + cc] = u;
+    });
 }
 
 // Rows pass of one block: inverse DFT along axis 1, in place, for the W
-// columns (p, c0..c0+W-1).  lds: W*(D+1) float2.
-template <int D1, int D2, int W>
+// columns (p, c0..c0+W-1).  lds: W*(D/ROUNDS+1) float2.
+template <int D1, int D2, int W, int ROUNDS = 1>
 __device__ __forceinline__ void rows_body(float2* __restrict__ T, int ho, const float2* __restrict__ tw,
                                           const FusedParams& fp, int p, int cgroup, float2* lds) {
-    constexpr int D = D1 * D2, H = D / 2 + 1, NT = W * D1, LD = D + 1;
+    constexpr int D = D1 * D2, H = D / 2 + 1;
     const int t = threadIdx.x;
     const int j = t % W, n1 = t / W;
     const int c0 = cgroup * W;
@@ -104,21 +128,10 @@ __device__ __forceinline__ void rows_body(float2* __restrict__ T, int ho, const 
         v[n2] = (c < H && bs * bs + c * c <= fp.m2max) ? T[((size_t)p * D + b) * ho + c] : make_float2(0.0f, 0.0f);
     }
     dft<D2>(v);
-#pragma unroll
-    for (int k2 = 0; k2 < D2; ++k2) lds[j * LD + k2 * D1 + n1] = (n1 * k2 == 0) ? v[k2] : cmul(v[k2], tw[n1 * k2]);
-    __syncthreads();
-    for (int it = t; it < W * D2; it += NT) {
-        const int jj = it % W, k2 = it / W;
+    transpose_finish<D1, D2, W, ROUNDS>(v, tw, lds, [&](int jj, int k2, int k1, float2 u) {
         const int cc = c0 + jj;
-        float2 u[D1];
-#pragma unroll
-        for (int n = 0; n < D1; ++n) u[n] = lds[jj * LD + k2 * D1 + n];
-        dft<D1>(u);
-        if (cc <= fp.cmax) {
-#pragma unroll
-            for (int k1 = 0; k1 < D1; ++k1) T[((size_t)p * D + (k2 + D2 * k1)) * ho + cc] = u[k1];
-        }
-    }
+        if (cc <= fp.cmax) T[((size_t)p * D + (k2 + D2 * k1)) * ho + cc] = u;
+    });
 }
 
 // c2r of L consecutive lines along the last axis.  The length-D real output
@@ -132,6 +145,7 @@ template <int N1, int N2, int L>
 struct LinesCore {
     static constexpr int N = N1 * N2, D = 2 * N, H = N + 1, NT = L * N1;
     static constexpr int HP = H | 1;
+    static constexpr int HO = padded_row(H);  // line stride of the intermediates
     static constexpr int LOADS = (L * H + NT - 1) / NT;
     static constexpr int ITEMS = (L * N2 + NT - 1) / NT;  // step-3 transforms per thread
     static constexpr int WORDS = L * D / 64;              // mask words of the block's cells

@@ -21,6 +21,10 @@ constexpr int kReduceBlocks = 1024;    // fixed grid for deterministic reduction
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Row length (complex elements) of the fused loop's intermediates: the D/2+1
+// modes of a half-spectrum line padded to whole 128-byte lines.
+__host__ __device__ constexpr int padded_row(int h) { return (h + 15) / 16 * 16; }
+
 inline unsigned grid_for(size_t n, int threads = kThreads) {
     return (unsigned)((n + threads - 1) / threads);
 }

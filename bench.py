@@ -29,14 +29,13 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
 def alg_bytes(D, norm=1):
-    """Algorithmic bytes per launch (one radius) of each stage, from the unit
-    model B_alg = 4N + 8Nc + n_R (24 Nc + (16 + 4 norm) N): filter 16Nc,
-    c2r 8Nc + 4N, f_coll box sum 4N, threshold 12N.  A fused kernel carries the
-    model bytes of the stages it replaces: columns = filter; lines = c2r plus
-    the box sum (nonlinear input) or plus the threshold (linear input); rows
-    are the middle of the c2r and carry none of their own; the paired launch
-    of nonlinear input carries the c2r and box sum of one radius and the
-    threshold of the previous one."""
+    """Algorithmic bytes per launch of each stage, from the unit model
+    B_alg = 4N + 8Nc + n_R (24 Nc + (16 + 4 norm) N): filter 16Nc,
+    c2r 8Nc + 4N, f_coll box sum 4N, threshold 12N.  A fused launch carries the
+    model bytes of the stages it replaces: one pipelined radius launch holds
+    one radius' worth of every stage (the filter in its columns role, the c2r
+    across its columns, rows and lines roles, the box sum and the threshold),
+    24Nc + (16 + 4 norm) N; the final paint carries the last threshold, 12N."""
     N = D ** 3
     Nc = D * D * (D // 2 + 1)
     return {
@@ -45,11 +44,8 @@ def alg_bytes(D, norm=1):
         "c2r": 8 * Nc + 4 * N,
         "fcoll_sum": 4 * N,
         "threshold": 12 * N,
-        "fused_columns": 16 * Nc,
-        "fused_rows": 0,
-        "fused_lines": 8 * Nc + 4 * N + (4 * N if norm else 12 * N),
-        "fused_lines_threshold": 12 * N,
-        "fused_lines_pair": 8 * Nc + 8 * N + 12 * N,
+        "fused_radius": 24 * Nc + (16 + 4 * norm) * N,
+        "fused_paint": 12 * N,
     }
 
 

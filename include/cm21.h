@@ -234,13 +234,13 @@ int cm21_power_spectrum(cm21_plan* plan, const float* field,
 #define CM21_STAGE_FCOLL_SUM 3
 #define CM21_STAGE_THRESHOLD 4
 #define CM21_STAGE_SPHERE 5
-#define CM21_STAGE_FUSED_COLUMNS 6
-#define CM21_STAGE_FUSED_ROWS 7
-#define CM21_STAGE_FUSED_LINES 8           /* the first lines pass of a radius (box sum for
-                                              nonlinear input, else the threshold/paint) */
-#define CM21_STAGE_FUSED_LINES_THRESHOLD 9 /* the threshold/paint lines pass after a box sum */
-#define CM21_STAGE_FUSED_LINES_PAIR 10     /* threshold pass of radius r with the box-sum pass
-                                              of radius r+1 in one launch (nonlinear input) */
+#define CM21_STAGE_FUSED_COLUMNS 6         /* batched: columns pass of every (radius, box) */
+#define CM21_STAGE_FUSED_ROWS 7            /* batched: rows pass */
+#define CM21_STAGE_FUSED_LINES 8           /* batched: persistent lines pass */
+#define CM21_STAGE_FUSED_LINES_THRESHOLD 9 /* single box: the final paint */
+#define CM21_STAGE_FUSED_RADIUS 10         /* single box: one launch per radius (columns, rows,
+                                              box sum, reduction and threshold roles of
+                                              neighbouring radii) */
 #define CM21_NSTAGES 11
 int cm21_profile_enable(int stage_mask);
 int cm21_profile_collect(double* ms, int64_t* count, int nstages);
