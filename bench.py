@@ -9,9 +9,18 @@ deposit, and the resulting nonlinear 256^3 box goes through find_HII_bubbles:
 normalisation, sigma_min^2 = 4.0, barrier 1.686/0.12.
 
 A step is one find_HII_bubbles pass over that box, with the box already in
-HBM.  The JSON line reports boxes/s, the roofline of the dominant stage
-(algorithmic bytes per launch over its HIP-event-timed average duration) and a
-CPU baseline: the NumPy FP32 oracle of the same algorithm on a bounded sample.
+HBM.  The JSON line reports boxes/s and, for the dominant kernel (the
+pipelined radius launch), its roofline by the unit byte model (algorithmic
+bytes per launch over its HIP-event-timed average duration; the fused loop
+moves far fewer bytes than the model counts, so the fraction can exceed 1),
+the calibrated HBM traffic per launch and the VALU issue rate (both from
+committed rocprofv3 counter summaries under profiles/).  Beside the headline:
+a partly ionized variant of the same box, the 512^3 and 64^3 (literal
+parameters) boxes, the 64-box batched path (literal and partly ionized), the
+U-Net surrogate pipeline with its validation, and a CPU baseline: the NumPy
+FP32 oracle of the same algorithm on a bounded sample (one core and all
+cores).  --no-extra / --no-batched / --no-surrogate / --no-cpu-baseline skip
+those parts.
 """
 import argparse
 import importlib
@@ -26,6 +35,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+# VALU issue peak of MI355X for wave64 FP32 instructions: 256 CUs x 4 SIMDs x
+# 2.4 GHz, four cycles per instruction (packed FP32 is not used)
+VALU_PEAK_G = 256 * 4 * 2.4 / 4
 
 
 def alg_bytes(D, norm=1):
@@ -585,6 +597,8 @@ def main():
     # calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of the same
     # kernel and workload, from the committed counter summary
     traffic = None if args.generic else read_traffic(f"{dom}_D{D}_traffic.txt", "bytes_per_launch")
+    # counter-measured VALU wave-instructions per launch of the same kernel
+    valu = None if args.generic else read_traffic(f"{dom}_D{D}_counters.md", "valu_per_launch")
 
     # delta_T and P21(k) in 40 log bins k_fund..k_nyq on the result (once)
     def post():
@@ -618,6 +632,12 @@ def main():
                      "traffic_gbs": (traffic / (dom_ms * 1e-3) / 1e9) if traffic else None,
                      "traffic_frac": (traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
                      "alg_bytes_per_launch": ab[dom], "ms_per_launch": dom_ms},
+        # what bounds the kernel: VALU issue (counter-measured instructions per
+        # launch over this run's launch time)
+        "valu_roofline": ({"achieved": valu / (dom_ms * 1e-3) / 1e9, "peak": VALU_PEAK_G,
+                           "unit": "G wave64-instructions/s", "frac": valu / (dom_ms * 1e-3) / 1e9 / VALU_PEAK_G,
+                           "valu_per_launch": valu, "source": f"profiles/{dom}_D{D}_counters.md"}
+                          if valu else None),
         "model_roofline": {"B_alg_per_box": b_alg(D, len(radii), 1),
                            "frac": b_alg(D, len(radii), 1) * boxes_s / (HBM_PEAK_GBS * 1e9)},
         "stages": stages_all,

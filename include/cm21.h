@@ -126,7 +126,8 @@ typedef struct cm21_bubble_params {
     int sphere_periodic;  /* sphere paint wraps (1) or clips (0) */
     float sphere_r_scale; /* R_cells = R * sphere_r_scale (1 = reference) */
     int fused;            /* 1 = fused pruned transforms where supported (update 0,
-                             filter_type 1 or 2, D in {64,96,128,160,256,512});
+                             filter_type 1 or 2, D in {64,96,128,160,256,512}):
+                             one launch per radius, every launch on `stream`;
                              0 = rocFFT c2r + separate filter/threshold kernels */
     int graph;            /* 1 = capture the loop into a HIP graph on the first call
                              and replay it for identical later calls (same
