@@ -598,7 +598,7 @@ def main():
     # kernel and workload, from the committed counter summary
     traffic = None if args.generic else read_traffic(f"{dom}_D{D}_traffic.txt", "bytes_per_launch")
     # counter-measured VALU wave-instructions per launch of the same kernel
-    valu = None if args.generic else read_traffic(f"{dom}_D{D}_counters.md", "valu_per_launch")
+    valu = None if args.generic else read_traffic(f"{dom}_D{D}_traffic.txt", "valu_per_launch")
 
     # delta_T and P21(k) in 40 log bins k_fund..k_nyq on the result (once)
     def post():
@@ -636,7 +636,8 @@ def main():
         # launch over this run's launch time)
         "valu_roofline": ({"achieved": valu / (dom_ms * 1e-3) / 1e9, "peak": VALU_PEAK_G,
                            "unit": "G wave64-instructions/s", "frac": valu / (dom_ms * 1e-3) / 1e9 / VALU_PEAK_G,
-                           "valu_per_launch": valu, "source": f"profiles/{dom}_D{D}_counters.md"}
+                           "valu_per_launch": valu,
+                           "source": f"profiles/{dom}_D{D}_traffic.txt (from profiles/{dom}_D{D}_counters.md)"}
                           if valu else None),
         "model_roofline": {"B_alg_per_box": b_alg(D, len(radii), 1),
                            "frac": b_alg(D, len(radii), 1) * boxes_s / (HBM_PEAK_GBS * 1e9)},
