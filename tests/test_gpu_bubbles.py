@@ -102,6 +102,49 @@ def test_loop_nonlinear(gpu, D, L, fused):
     _compare(got, want, margin)
 
 
+@pytest.mark.parametrize("LE", [True, False])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6])
+def test_loop_short_radius_lists(gpu, n, LE):
+    """The fused loop with 1..6 radii through the C ABI: one launch per
+    radius carries the roles of radii r-2 .. r+2, so short lists exercise the
+    pipeline's start and end overlapping (a launch with only the columns role,
+    only the reduction and threshold roles, ...).  Radii and denominators are
+    the first n of the schedule; the last of them is the final radius."""
+    lib = pkg("_lib")
+    D, L = 64, 100.0
+    if LE:
+        delta = grf.c2r(grf.grf_half_spectrum(D, L, 1), D).astype(np.float32)
+        deltac, smin, gi = 1.686, 0.5, 1.0
+    else:
+        delta = (grf.c2r(grf.grf_half_spectrum(D, L, 2), D) * np.float32(0.12)).astype(np.float32)
+        deltac, smin = 1.686 / 0.12, 44.0
+        gi = float(np.float32(deltac / 1.686))
+    radii_all, _ = sigma.radius_schedule(L, D)
+    den_all = sigma.denominators(radii_all, smin, lambda R: sigma.sig0_discrete(R, D, L))
+    radii = np.ascontiguousarray(radii_all[:n], np.float32)
+    den = np.ascontiguousarray(den_all[:n], np.float32)
+    fc_mean = np.float32(erfc(deltac / np.sqrt(2 * smin)))
+    prm = lib.BubbleParams(box_len=L, deltac=float(np.float32(deltac)), zeta=30.0, growth_inv=gi,
+                           fc_mean=float(fc_mean), linear_input=1 if LE else 0, filter_type=1, ref_fallthrough=1,
+                           update=0, sphere_periodic=0, sphere_r_scale=1.0, fused=1, graph=0)
+    plan = lib.plan(D, 1)
+    d_dev = torch.from_numpy(delta).cuda()
+    ion = torch.full_like(d_dev, -7.0)
+    nbytes = lib.query("cm21_find_bubbles_workspace_bytes", D, n, 0)
+    work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    lib.call("cm21_find_bubbles", plan.handle, lib.ptr(d_dev), lib.ptr(ion), radii.ctypes.data_as(lib.c_float_p),
+             den.ctypes.data_as(lib.c_float_p), n, prm, lib.ptr(work), nbytes,
+             lib.stream_handle(torch.cuda.current_stream()))
+    got = ion.cpu().numpy()
+    want = bubbles.find_bubbles_f32(delta, L, radii, den, 30.0, deltac, LE=LE, growth_inv=np.float32(gi),
+                                    fc_mean=fc_mean)
+    _, margin = bubbles.find_bubbles_f64(delta, L, radii, den, 30.0, deltac, LE=LE, growth_inv=deltac / 1.686,
+                                         fc_mean=float(fc_mean))
+    print(f"n={n} LE={LE}: ionized fraction {np.mean(want == 1):.4f}", flush=True)
+    assert np.all(got >= 0)
+    _compare(got, want, margin)
+
+
 @pytest.mark.parametrize("periodic,in_cells", [(False, False), (True, True), (False, True)])
 def test_loop_sphere_paint(gpu, periodic, in_cells):
     D, L = 32, 100.0
