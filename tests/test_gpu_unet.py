@@ -381,22 +381,52 @@ def test_trained_network_gains_over_untrained(gpu):
 
 
 def test_pipeline_validates_and_falls_back(gpu):
-    """Served with the trained network, sampled boxes are validated against
-    the plain method; an untrained network fails validation, its requests
-    are answered by the plain method, and after max_failures it is bypassed."""
+    """Served with the trained network at the production tolerances (|dx_HI| <
+    0.01, P21 within 10 %): the validated box decides the answer.  The
+    network's prediction and its metrics against the plain method are computed
+    independently first; a pipeline whose sampled box meets the tolerances
+    answers with the network's fields, one whose sampled box misses them
+    answers the whole request with the plain fields.  An untrained network
+    fails validation, and after max_failures it is bypassed."""
     sur = pkg("surrogate")
     params, _ = sur.UNet3D.load_file(sur.WEIGHTS)
-    boxes = _heldout(4)
-    zeta = [s["zeta"] for s in boxes.specs]
-    smin = [s["sigma_min2"] for s in boxes.specs]
-    good = sur.SurrogatePipeline(sur.UNet3D(D=128, batch=4, params=params), validate_every=2, tol_xhi=0.05,
-                                 tol_p21=10.0)
-    fields, src = good(boxes.delta, zeta, smin)
-    assert good.validated == 2
-    assert src == "surrogate" and good.fallbacks == 0
-    bad = sur.SurrogatePipeline(sur.UNet3D(D=128, batch=4, seed=5), validate_every=1, tol_xhi=0.01, max_failures=2)
+    boxes = _heldout(8)
+    model = sur.UNet3D(D=128, batch=4, params=params)
+    n_surrogate = n_plain = 0
+    for first in (0, 4):
+        delta = boxes.delta[first:first + 4].contiguous()
+        zeta = [s["zeta"] for s in boxes.specs[first:first + 4]]
+        smin = [s["sigma_min2"] for s in boxes.specs[first:first + 4]]
+        cond = [sur.normalise_cond(z, s) for z, s in zip(zeta, smin)]
+        prob = model.forward(delta, cond, keep=False).clone()
+        metrics = [sur.box_metrics(prob[j], boxes.target[first + j], delta[j], 200.0) for j in range(4)]
+        for j in range(4):
+            m = metrics[j]
+            ok = m["dx_HI"] <= 0.01 and m["p21_err"] <= 0.10
+            print(f"box {first + j}: dx_HI {m['dx_HI']:.4f} P21 {m['p21_err']:.3f} -> "
+                  f"{'within' if ok else 'outside'} the tolerances", flush=True)
+            pipe = sur.SurrogatePipeline(model, validate_every=4)  # production tolerances
+            assert (pipe.tol_xhi, pipe.tol_p21) == (0.01, 0.10)
+            pipe.seen = (4 - j) % 4  # the request's box j is the sampled one
+            fields, src = pipe(delta, zeta, smin)
+            assert pipe.validated == 1
+            if ok:
+                n_surrogate += 1
+                assert src == "surrogate" and pipe.fallbacks == 0
+                assert torch.equal(fields, prob)
+            else:
+                n_plain += 1
+                assert src == "plain" and pipe.fallbacks == 1 and pipe.failures == 1
+                for i in range(4):
+                    assert torch.equal(fields[i], boxes.target[first + i]), (first, j, i)
+    print(f"answered by the network {n_surrogate}, by the plain method {n_plain}", flush=True)
+    assert n_surrogate > 0  # the trained network passes on some held-out boxes
+    delta = boxes.delta[:4].contiguous()
+    zeta = [s["zeta"] for s in boxes.specs[:4]]
+    smin = [s["sigma_min2"] for s in boxes.specs[:4]]
+    bad = sur.SurrogatePipeline(sur.UNet3D(D=128, batch=4, seed=5), validate_every=1, max_failures=2)
     for rep in range(3):
-        fields, src = bad(boxes.delta, zeta, smin)
+        fields, src = bad(delta, zeta, smin)
         assert src == "plain"
         for j in range(4):
             assert torch.equal(fields[j], boxes.target[j]), (rep, j)
