@@ -454,6 +454,7 @@ int run_loop(cm21_plan* plan, const float* delta, float* ionized, const float* r
             }
             CM21_TRY(fused_radius_launch(*sh, a, tw, st));
         }
+        // mode 2: the final paint of the last radius
         CM21_TRY(fused_lines(*sh, 2, w.inter[last % kInter], tw, params(last), threshold_epilogue(last), st,
                              CM21_STAGE_FUSED_LINES_THRESHOLD));
         return CM21_OK;
