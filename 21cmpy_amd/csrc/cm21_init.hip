@@ -115,7 +115,9 @@ __device__ __forceinline__ void grf_mode(float2* __restrict__ spec, int D, int a
         amp = sqrtf(table_pk(k, sK, sP, gp.ntab) * gp.inv_volume);
     } else {
         const float x = k * gp.inv_k_cut;
-        amp = sqrtf(gp.amp_over_volume * (powf(k, gp.n_indexf) * expf(-(x * x))));
+        // k^n as exp2(n log2 k) (k > 0 here): about 1e-6 relative, a third of
+        // the instructions of powf
+        amp = sqrtf(gp.amp_over_volume * (exp2f(gp.n_indexf * log2f(k)) * expf(-(x * x))));
     }
     if (selfconj) {
         spec[idx] = make_float2(amp * g1, 0.0f);
@@ -126,8 +128,9 @@ __device__ __forceinline__ void grf_mode(float2* __restrict__ spec, int D, int a
 }
 
 // Blocks of 64 x 4 threads; block (0, y, z) covers a = z and the kGrfRows
-// rows b from y * kGrfRows, each thread walking c by 64 and b by 4, so the
-// P(k) table is loaded into LDS once per 4 * kGrfRows * H modes.
+// rows b from y * kGrfRows, its 256 threads walking the rows' modes as one
+// run (no lane idles at the end of each row of D/2+1 modes), so the P(k)
+// table is loaded into LDS once per kGrfRows * H modes.
 constexpr int kGrfRows = 16;
 
 __global__ __launch_bounds__(kModeBlockX* kModeBlockY) void k_grf(float2* __restrict__ spec, int D, GrfParams gp) {
@@ -142,9 +145,12 @@ __global__ __launch_bounds__(kModeBlockX* kModeBlockY) void k_grf(float2* __rest
     }
     const int H = D / 2 + 1;
     const int a = blockIdx.z;
-    const int b_end = min(D, (int)(blockIdx.y + 1) * kGrfRows);
-    for (int b = blockIdx.y * kGrfRows + threadIdx.y; b < b_end; b += kModeBlockY)
-        for (int c = threadIdx.x; c < H; c += kModeBlockX) grf_mode(spec, D, a, b, c, gp, sK, sP);
+    const int b0 = blockIdx.y * kGrfRows;
+    const int n = (min(D, b0 + kGrfRows) - b0) * H;  // the block's modes, rows b0.. of plane a
+    for (int q = threadIdx.y * kModeBlockX + threadIdx.x; q < n; q += kModeBlockX * kModeBlockY) {
+        const int rb = q / H;
+        grf_mode(spec, D, a, b0 + rb, q - rb * H, gp, sK, sP);
+    }
 }
 
 __global__ void k_subsample(float* __restrict__ small, const float* __restrict__ large, int D, int sw, float pf) {
