@@ -360,14 +360,17 @@ int run_loop(cm21_plan* plan, const float* delta, float* ionized, const float* r
 
     const float dk = (float)(2.0 * M_PI / (double)prm->box_len);
     const float scale = prm->linear_input ? (float)(1.0 / (double)N) : (float)((double)prm->growth_inv / (double)N);
-    {
-        ProfToken t = prof_begin(CM21_STAGE_R2C, st);
-        CM21_TRY(cm21_r2c(plan, delta, w.spec, stream));
-        prof_end(t, st);
-    }
     const FusedShape* sh = fused_shape(D);
     if (prm->fused && sh && prm->update == 0 && (prm->filter_type == 1 || prm->filter_type == 2)) {
         const float2* tw = plan_twiddles(plan);
+        // the forward transform forms only the modes the widest window keeps,
+        // through the intermediate that radius 4 uses first
+        FusedParams widest = fused_params(D, dk, radii[0], prm->filter_type, prm->ref_fallthrough, scale);
+        for (int r = 1; r < n_radii; ++r) {
+            const FusedParams f = fused_params(D, dk, radii[r], prm->filter_type, prm->ref_fallthrough, scale);
+            if (f.m2max > widest.m2max) widest = f;
+        }
+        CM21_TRY(fused_forward(*sh, delta, w.inter[kInter - 1], w.spec, tw, widest, st));
         CM21_TRY(fused_mask_clear(w.mask, (N + 63) / 64, st));
         // One launch per radius on the caller's stream, each with up to five
         // independent roles over neighbouring radii (k_fused_radius): launch
@@ -458,6 +461,11 @@ int run_loop(cm21_plan* plan, const float* delta, float* ionized, const float* r
         CM21_TRY(fused_lines(*sh, 2, w.inter[last % kInter], tw, params(last), threshold_epilogue(last), st,
                              CM21_STAGE_FUSED_LINES_THRESHOLD));
         return CM21_OK;
+    }
+    {
+        ProfToken t = prof_begin(CM21_STAGE_R2C, st);
+        CM21_TRY(cm21_r2c(plan, delta, w.spec, stream));
+        prof_end(t, st);
     }
     k_fill<<<grid_for(N), kThreads, 0, st>>>(ionized, N, 0.0f);
     CM21_LAUNCH_CHECK("k_fill");

@@ -90,7 +90,7 @@ __device__ __forceinline__ void lines_block(const float2* __restrict__ U, const 
                                             const FusedParams& fp, const LineEpilogue& ep, int blk, float2* buf,
                                             float2* stw, unsigned long long* smask) {
     using C = LinesCore<N1, N2, L>;
-    constexpr int D = C::D, H = C::H, NT = C::NT, WORDS = C::WORDS;
+    constexpr int D = C::D, NT = C::NT, WORDS = C::WORDS;
     const int t = threadIdx.x;
     const size_t line0 = (size_t)blk * L;
     const size_t word0 = line0 * D / 64;
@@ -256,6 +256,52 @@ __global__ __launch_bounds__(W* D1, 8) void k_fused_radius(RadiusRoles a, const 
     }
 }
 
+// Pruned r2c of the fused loop (three launches): only the modes that the
+// smallest radius keeps (|n|^2 <= m2max, c <= cmax) are formed.
+// Lines: the r2c along c of L lines per block, from the real box into T0
+// (stride HO, c <= cmax).
+template <int N1, int N2, int L>
+__global__ __launch_bounds__(L* N1) void k_fwd_lines(const float* __restrict__ delta, float2* __restrict__ T0,
+                                                    const float2* __restrict__ tw, int cmax) {
+    using C = LinesCore<N1, N2, L>;
+    constexpr int N = C::N, D = C::D, NT = C::NT, HP = C::HP;
+    __shared__ float2 buf[L * HP];
+    __shared__ float2 stw[D];
+    const int t = threadIdx.x;
+    const size_t line0 = (size_t)blockIdx.x * L;
+    const float2* src = reinterpret_cast<const float2*>(delta + line0 * D);
+    for (int f = t; f < L * N; f += NT) {
+        const int i = f / N, m = f - i * N;
+        buf[i * HP + m] = src[f];
+    }
+    for (int q = t; q < D; q += NT) stw[q] = tw[q];
+    __syncthreads();
+    forward_pairs<N1, N2, L>(buf, stw);
+    const int nk = cmax + 1;
+    for (int f = t; f < L * nk; f += NT) {
+        const int i = f / nk, k = f - i * nk;
+        const float2 Wk = buf[i * HP + (k == N ? 0 : k)];
+        const float2 Wmk = buf[i * HP + (k == 0 ? 0 : N - k)];
+        T0[(line0 + i) * C::HO + k] = line_spectrum(Wk, Wmk, stw[k]);
+    }
+}
+
+template <int D1, int D2, int W>
+__global__ __launch_bounds__(W* D1) void k_fwd_rows(float2* __restrict__ T0, const float2* __restrict__ tw, int cmax,
+                                                   int m2max) {
+    constexpr int D = D1 * D2;
+    __shared__ float2 lds[W * (D + 1)];
+    fwd_rows_body<D1, D2, W>(T0, padded_row(D / 2 + 1), tw, cmax, m2max, blockIdx.y, blockIdx.x, lds);
+}
+
+template <int D1, int D2, int W>
+__global__ __launch_bounds__(W* D1) void k_fwd_columns(const float2* __restrict__ T0, float2* __restrict__ S,
+                                                      const float2* __restrict__ tw, int bm, int cmax, int m2max) {
+    constexpr int D = D1 * D2;
+    __shared__ float2 lds[W * (D + 1)];
+    fwd_columns_body<D1, D2, W>(T0, padded_row(D / 2 + 1), S, tw, bm, cmax, m2max, blockIdx.y, blockIdx.x, lds);
+}
+
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
@@ -288,14 +334,31 @@ int launch_radius(const RadiusRoles& a, const float2* tw, hipStream_t st) {
     return CM21_OK;
 }
 
-// D, lines (N1, N2, L), radius (columns/rows D1, D2, W; lines N1, N2, L)
+template <int D1, int D2, int W, int N1, int N2, int L>
+int launch_forward(const float* delta, float2* T0, float2* S, const float2* tw, const FusedParams& fp,
+                   hipStream_t st) {
+    constexpr int D = D1 * D2, H = D / 2 + 1;
+    const int cmax = fp.cmax < H - 1 ? fp.cmax : H - 1;
+    k_fwd_lines<N1, N2, L><<<D * D / L, L * N1, 0, st>>>(delta, T0, tw, cmax);
+    CM21_LAUNCH_CHECK("k_fwd_lines");
+    const int ncg = cmax / W + 1;
+    k_fwd_rows<D1, D2, W><<<dim3(ncg, D), W * D1, 0, st>>>(T0, tw, cmax, fp.m2max);
+    CM21_LAUNCH_CHECK("k_fwd_rows");
+    const int bpos = fp.bm < D / 2 ? fp.bm : D / 2;
+    const int bneg = fp.bm < D / 2 - 1 ? fp.bm : D / 2 - 1;
+    k_fwd_columns<D1, D2, W><<<dim3(ncg, bpos + 1 + bneg), W * D1, 0, st>>>(T0, S, tw, fp.bm, cmax, fp.m2max);
+    CM21_LAUNCH_CHECK("k_fwd_columns");
+    return CM21_OK;
+}
+
+// D, lines (N1, N2, L), radius (columns/rows D1, D2, W; lines N1, N2, L), forward
 const FusedShape kShapes[] = {
-    {64, launch_lines<4, 8, 64>, launch_radius<8, 8, 32, 4, 8, 64>, 32, 64},
-    {96, launch_lines<4, 12, 64>, launch_radius<8, 12, 32, 4, 12, 64>, 32, 64},
-    {128, launch_lines<8, 8, 32>, launch_radius<8, 16, 32, 8, 8, 32>, 32, 32},
-    {160, launch_lines<8, 10, 32>, launch_radius<16, 10, 16, 8, 10, 32>, 16, 32},
-    {256, launch_lines<16, 8, 16>, launch_radius<16, 16, 16, 16, 8, 16>, 16, 16},
-    {512, launch_lines<16, 16, 8>, launch_radius<16, 32, 8, 16, 16, 8>, 8, 8},
+    {64, launch_lines<4, 8, 64>, launch_radius<8, 8, 32, 4, 8, 64>, launch_forward<8, 8, 32, 4, 8, 64>, 32, 64},
+    {96, launch_lines<4, 12, 64>, launch_radius<8, 12, 32, 4, 12, 64>, launch_forward<8, 12, 32, 4, 12, 64>, 32, 64},
+    {128, launch_lines<8, 8, 32>, launch_radius<8, 16, 32, 8, 8, 32>, launch_forward<8, 16, 32, 8, 8, 32>, 32, 32},
+    {160, launch_lines<8, 10, 32>, launch_radius<16, 10, 16, 8, 10, 32>, launch_forward<16, 10, 16, 8, 10, 32>, 16, 32},
+    {256, launch_lines<16, 8, 16>, launch_radius<16, 16, 16, 16, 8, 16>, launch_forward<16, 16, 16, 16, 8, 16>, 16, 16},
+    {512, launch_lines<16, 16, 8>, launch_radius<16, 32, 8, 16, 16, 8>, launch_forward<16, 32, 8, 16, 16, 8>, 8, 8},
 };
 
 const FusedShape* fused_shape(int D) {
@@ -359,6 +422,14 @@ int fused_lines(const FusedShape& sh, int mode, const float2* U, const float2* t
                 const LineEpilogue& ep, hipStream_t st, int stage) {
     ProfToken t = prof_begin(stage, st);
     CM21_TRY(sh.lines(mode, U, tw, fp, ep, st));
+    prof_end(t, st);
+    return CM21_OK;
+}
+
+int fused_forward(const FusedShape& sh, const float* delta, float2* T0, float2* S, const float2* tw,
+                  const FusedParams& fp, hipStream_t st) {
+    ProfToken t = prof_begin(CM21_STAGE_R2C, st);
+    CM21_TRY(sh.forward(delta, T0, S, tw, fp, st));
     prof_end(t, st);
     return CM21_OK;
 }

@@ -96,6 +96,7 @@ struct FusedShape {
     int D;
     int (*lines)(int mode, const float2*, const float2*, const FusedParams&, const LineEpilogue&, hipStream_t);
     int (*radius)(const RadiusRoles&, const float2* tw, hipStream_t);
+    int (*forward)(const float* delta, float2* T0, float2* S, const float2* tw, const FusedParams& fp, hipStream_t);
     int W, L;  // columns per column group, lines per line group
 };
 
@@ -109,6 +110,12 @@ LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double*
 // (the rows role uses the same column groups over all D rows p).
 void column_groups(const FusedShape& sh, const FusedParams& fp, int* ncg, int* nb);
 int fused_radius_launch(const FusedShape& sh, const RadiusRoles& a, const float2* tw, hipStream_t st);
+// The r2c of delta into the half spectrum S for the fused loop, formed only
+// for the modes the widest window fp keeps (|n|^2 <= fp.m2max), through the
+// intermediate T0 (rows padded to 16 modes).  S entries outside that sphere
+// are left untouched: no radius reads them.
+int fused_forward(const FusedShape& sh, const float* delta, float2* T0, float2* S, const float2* tw,
+                  const FusedParams& fp, hipStream_t st);
 int fused_lines(const FusedShape& sh, int mode, const float2* U, const float2* tw, const FusedParams& fp,
                 const LineEpilogue& ep, hipStream_t st, int stage = CM21_STAGE_FUSED_LINES);
 int fused_mask_clear(unsigned long long* mask, size_t words, hipStream_t st);

@@ -134,6 +134,61 @@ __device__ __forceinline__ void rows_body(float2* __restrict__ T, int ho, const 
     });
 }
 
+// Forward DFT along a column axis for the pruned r2c: the inverse four-step
+// of the conjugated inputs, conjugated on the way out (forward(x) =
+// conj(inverse(conj x))).  Rows: plane p, in place along b, every b loaded
+// for c <= cmax, stored only where b^2 + c^2 <= m2max (the columns pass reads
+// nothing else).  Columns: T (rows layout, stride ho) along a into the
+// half spectrum S (stride D/2+1), only columns inside the disc and modes
+// inside the sphere |n|^2 <= m2max (no radius reads beyond it).
+template <int D1, int D2, int W>
+__device__ __forceinline__ void fwd_rows_body(float2* __restrict__ T, int ho, const float2* __restrict__ tw, int cmax,
+                                              int m2max, int p, int cgroup, float2* lds) {
+    constexpr int D = D1 * D2;
+    const int t = threadIdx.x;
+    const int j = t % W, n1 = t / W;
+    const int c0 = cgroup * W;
+    const int c = c0 + j;
+    float2 v[D2];
+#pragma unroll
+    for (int n2 = 0; n2 < D2; ++n2) {
+        const int b = n1 + D1 * n2;
+        v[n2] = c <= cmax ? cconj(T[((size_t)p * D + b) * ho + c]) : make_float2(0.0f, 0.0f);
+    }
+    dft<D2>(v);
+    transpose_finish<D1, D2, W, 1>(v, tw, lds, [&](int jj, int k2, int k1, float2 u) {
+        const int cc = c0 + jj, b = k2 + D2 * k1, bs = mode_of(b, D);
+        if (cc <= cmax && bs * bs + cc * cc <= m2max) T[((size_t)p * D + b) * ho + cc] = cconj(u);
+    });
+}
+
+template <int D1, int D2, int W>
+__device__ __forceinline__ void fwd_columns_body(const float2* __restrict__ T, int ho, float2* __restrict__ S,
+                                                 const float2* __restrict__ tw, int bm, int cmax, int m2max, int slot,
+                                                 int cgroup, float2* lds) {
+    constexpr int D = D1 * D2, H = D / 2 + 1;
+    const int t = threadIdx.x;
+    const int j = t % W, n1 = t / W;
+    const int bpos = bm < D / 2 ? bm : D / 2;
+    const int b = slot <= bpos ? slot : D - (slot - bpos);
+    const int bs = mode_of(b, D);
+    const int c0 = cgroup * W;
+    if (bs * bs + c0 * c0 > m2max) return;  // whole group outside the disc (uniform)
+    const int c = c0 + j;
+    const bool col_ok = c <= cmax && bs * bs + c * c <= m2max;
+    float2 v[D2];
+#pragma unroll
+    for (int n2 = 0; n2 < D2; ++n2) {
+        const int a = n1 + D1 * n2;
+        v[n2] = col_ok ? cconj(T[((size_t)a * D + b) * ho + c]) : make_float2(0.0f, 0.0f);
+    }
+    dft<D2>(v);
+    transpose_finish<D1, D2, W, 1>(v, tw, lds, [&](int jj, int k2, int k1, float2 u) {
+        const int cc = c0 + jj, a = k2 + D2 * k1, as = mode_of(a, D);
+        if (cc <= cmax && as * as + bs * bs + cc * cc <= m2max) S[((size_t)a * D + b) * H + cc] = cconj(u);
+    });
+}
+
 // c2r of L consecutive lines along the last axis.  The length-D real output
 // is the length-N = D/2 complex transform of
 // Z[k] = (X[k] + conj X[N-k]) + i (X[k] - conj X[N-k]) w_D^k, whose outputs
@@ -220,6 +275,63 @@ struct LinesCore {
         __syncthreads();
     }
 };
+
+// Forward transform of real lines, for the pruned r2c of the fused loop.
+// buf holds L lines of D reals as N = D/2 packed pairs z[m] = (x[2m],
+// x[2m+1]) at buf[i*HP + m] (after a barrier); stw holds w_D^q, q < D.  The inverse four-step DFT of
+// conj(z) leaves W = conj(Z), Z the forward length-N DFT of z, at
+// buf[i*HP + k] (followed by a barrier); line_spectrum() then forms the
+// half-spectrum mode X[k] = (Z[k] + conj Z[N-k])/2 - i (Z[k] - conj Z[N-k])
+// w_D^-k / 2 for 0 <= k <= N.
+template <int N1, int N2, int L>
+__device__ __forceinline__ void forward_pairs(float2* buf, const float2* stw) {
+    constexpr int N = N1 * N2, HP = (N + 1) | 1, NT = L * N1;
+    constexpr int ITEMS = (L * N2 + NT - 1) / NT;
+    const int t = threadIdx.x;
+    {
+        const int i = t % L, n1 = t / L;
+        float2 v[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) v[n2] = cconj(buf[i * HP + n1 + N1 * n2]);
+        dft<N2>(v);
+        __syncthreads();
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2)
+            buf[i * HP + k2 * N1 + n1] = (n1 * k2 == 0) ? v[k2] : cmul(v[k2], stw[2 * n1 * k2]);
+    }
+    __syncthreads();
+    float2 u[ITEMS][N1];
+#pragma unroll
+    for (int q = 0; q < ITEMS; ++q) {
+        const int it = t + q * NT;
+        const int ii = it % L, k2 = it / L;
+        if (it < L * N2) {
+#pragma unroll
+            for (int n = 0; n < N1; ++n) u[q][n] = buf[ii * HP + k2 * N1 + n];
+            dft<N1>(u[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < ITEMS; ++q) {
+        const int it = t + q * NT;
+        const int ii = it % L, k2 = it / L;
+        if (it < L * N2) {
+#pragma unroll
+            for (int k1 = 0; k1 < N1; ++k1) buf[ii * HP + k2 + N2 * k1] = u[q][k1];
+        }
+    }
+    __syncthreads();
+}
+
+// X[k] of a line from W = conj(Z) (see forward_pairs); wk = w_D^k.
+__device__ __forceinline__ float2 line_spectrum(float2 Wk, float2 Wmk, float2 wk) {
+    const float2 zk = cconj(Wk);                // Z[k]
+    const float2 s = cadd(zk, Wmk);             // Z[k] + conj Z[N-k]
+    const float2 d = csub(zk, Wmk);             // Z[k] - conj Z[N-k]
+    const float2 e = cmul(d, cconj(wk));        // ... w_D^-k
+    return make_float2(0.5f * (s.x + e.y), 0.5f * (s.y - e.x));  // (s - i e) / 2
+}
 
 __device__ inline float fcoll_of(float smoothed, float deltac, float denom) {
     return erfcf(__fdiv_rn(__fsub_rn(deltac, smoothed), denom));
