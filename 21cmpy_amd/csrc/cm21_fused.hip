@@ -248,7 +248,7 @@ __global__ __launch_bounds__(W* D1, 8) void k_fused_radius(RadiusRoles a, const 
     if (!mem) {
         lines_block<N1, N2, L, kLineSum>(a.Usum, tw, a.fp_sum, a.ep_sum, i, buf, stw, smask);
     } else if (i < a.n_cols) {
-        columns_body<D1, D2, W, kRounds>(a.S, a.Tcols, C::HO, tw, a.fp_cols, a.fp_cols.scale, i / a.ncg_cols,
+        columns_body<D1, D2, W, kRounds, true>(a.S, a.Tcols, C::HO, tw, a.fp_cols, a.fp_cols.scale, i / a.ncg_cols,
                                 i % a.ncg_cols, lds);
     } else {
         i -= a.n_cols;

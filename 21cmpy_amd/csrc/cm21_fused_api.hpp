@@ -110,10 +110,11 @@ LineEpilogue line_epilogue(float deltac, float denom, double zeta, const double*
 // (the rows role uses the same column groups over all D rows p).
 void column_groups(const FusedShape& sh, const FusedParams& fp, int* ncg, int* nb);
 int fused_radius_launch(const FusedShape& sh, const RadiusRoles& a, const float2* tw, hipStream_t st);
-// The r2c of delta into the half spectrum S for the fused loop, formed only
-// for the modes the widest window fp keeps (|n|^2 <= fp.m2max), through the
-// intermediate T0 (rows padded to 16 modes).  S entries outside that sphere
-// are left untouched: no radius reads them.
+// The r2c of delta into the half spectrum S for the fused loop, stored
+// b-major (S[b][a][c]), formed only for the modes the widest window fp keeps
+// (|n|^2 <= fp.m2max), through the intermediate T0 (rows padded to 16
+// modes).  S entries outside that sphere are left untouched: no radius reads
+// them.
 int fused_forward(const FusedShape& sh, const float* delta, float2* T0, float2* S, const float2* tw,
                   const FusedParams& fp, hipStream_t st);
 int fused_lines(const FusedShape& sh, int mode, const float2* U, const float2* tw, const FusedParams& fp,
