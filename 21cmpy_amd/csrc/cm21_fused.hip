@@ -203,11 +203,13 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
 // then the columns/rows blocks interleaved one for one with the box-sum
 // blocks, so that the memory-bound transposes and the VALU-bound box sums
 // share every CU.
-// Eight waves per SIMD: the box-sum role (about 52 VGPRs) keeps full
-// occupancy; the columns/rows roles spill a few registers to scratch instead,
-// which costs them less than half the occupancy would cost the box sums.
-template <int D1, int D2, int W, int N1, int N2, int L>
-__global__ __launch_bounds__(W* D1, 8) void k_fused_radius(RadiusRoles a, const float2* __restrict__ tw) {
+// Two builds differ in occupancy: WAVES = 8 lets the box-sum role (about 52
+// VGPRs) run at full occupancy while the columns/rows roles spill registers
+// to scratch; WAVES = 4 gives the columns/rows roles their registers.  The
+// host picks the first while the box sums dominate a launch and the second
+// once the transposes do (the smallest radii).
+template <int D1, int D2, int W, int N1, int N2, int L, int WAVES>
+__global__ __launch_bounds__(W* D1, WAVES) void k_fused_radius(RadiusRoles a, const float2* __restrict__ tw) {
     using C = LinesCore<N1, N2, L>;
     constexpr int D = D1 * D2, NT = W * D1;
     static_assert(C::D == D && C::NT == NT, "columns/rows and lines tiles must share D and the block size");
@@ -325,11 +327,18 @@ int launch_lines(int mode, const float2* U, const float2* tw, const FusedParams&
     return CM21_OK;
 }
 
+// A launch whose columns/rows blocks exceed 1/kMemHeavy of its box-sum
+// blocks takes the low-occupancy build.
+constexpr int kMemHeavy = 2;
+
 template <int D1, int D2, int W, int N1, int N2, int L>
 int launch_radius(const RadiusRoles& a, const float2* tw, hipStream_t st) {
     const int blocks = a.n_red + a.n_thr + a.n_mem + a.n_sum;
     if (blocks <= 0) return CM21_OK;
-    k_fused_radius<D1, D2, W, N1, N2, L><<<blocks, W * D1, 0, st>>>(a, tw);
+    if (a.n_mem * kMemHeavy > a.n_sum)
+        k_fused_radius<D1, D2, W, N1, N2, L, 4><<<blocks, W * D1, 0, st>>>(a, tw);
+    else
+        k_fused_radius<D1, D2, W, N1, N2, L, 8><<<blocks, W * D1, 0, st>>>(a, tw);
     CM21_LAUNCH_CHECK("k_fused_radius");
     return CM21_OK;
 }
