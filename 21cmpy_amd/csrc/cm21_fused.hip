@@ -205,18 +205,21 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
 // share every CU.
 // Two builds differ in occupancy: WAVES = 8 lets the box-sum role (about 52
 // VGPRs) run at full occupancy while the columns/rows roles spill registers
-// to scratch; WAVES = 4 gives the columns/rows roles their registers.  The
-// host picks the first while the box sums dominate a launch and the second
-// once the transposes do (the smallest radii).
+// to scratch; WAVES = 4 gives the columns/rows roles their registers and the
+// LDS of a one-round transpose.  The host picks the first while the box sums
+// dominate a launch and the second once the transposes do (the smallest
+// radii).
 template <int D1, int D2, int W, int N1, int N2, int L, int WAVES>
 __global__ __launch_bounds__(W* D1, WAVES) void k_fused_radius(RadiusRoles a, const float2* __restrict__ tw) {
     using C = LinesCore<N1, N2, L>;
     constexpr int D = D1 * D2, NT = W * D1;
     static_assert(C::D == D && C::NT == NT, "columns/rows and lines tiles must share D and the block size");
-    // the columns/rows transposes run in two rounds when their one-round
-    // tile would need more LDS than the lines pass
+    // the eight-wave build runs the columns/rows transposes in two rounds
+    // when their one-round tile would need more LDS than the lines pass (and
+    // so lower the occupancy of the box sums); the four-wave build has the
+    // LDS for one round
     constexpr int kLdsLines = L * C::HP + D + C::WORDS;
-    constexpr int kRounds = W * (D + 1) > kLdsLines + kLdsLines / 4 ? 2 : 1;
+    constexpr int kRounds = (WAVES > 4 && W * (D + 1) > kLdsLines + kLdsLines / 4) ? 2 : 1;
     constexpr int kLdsCR = W * (D / kRounds + 1);
     __shared__ float2 lds[kLdsCR > kLdsLines ? kLdsCR : kLdsLines];
     float2* buf = lds;
