@@ -127,12 +127,17 @@ __device__ __forceinline__ void lines_block(const float2* __restrict__ U, const 
     const int lane = t & 63;
     const float* xrf = reinterpret_cast<const float*>(buf);
     const float inv_denom = 1.0f / ep.denom;
+    // box sum: four float32 partials per thread (a quarter of its cells each,
+    // a few roundings of values in [0, 2]), added in FP64 after the loop
+    static_assert((L * D) % (4 * NT) == 0, "whole sets of four cells per thread");
+    float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 4
-    for (int f = t; f < L * D; f += NT) {
+    for (int q = 0; q < L * D / NT; ++q) {
+        const int f = t + q * NT;
         const int i = f / D, r = f - i * D;
         const float delta = xrf[i * 2 * C::HP + r];
         if (MODE == kLineSum) {
-            acc += (double)fcoll_for_sum(delta, ep.deltac, ep.denom, inv_denom);
+            part[q & 3] += fcoll_for_sum(delta, ep.deltac, ep.denom, inv_denom);
             vmax = fmaxf(vmax, delta);
         } else {
             const bool hit = cell_hit(delta, th, ep.deltac, ep.denom, use_scale, scale, ep.inv_zeta);
@@ -149,6 +154,7 @@ __device__ __forceinline__ void lines_block(const float2* __restrict__ U, const 
         }
     }
     if (MODE == kLineSum) {
+        acc = ((double)part[0] + (double)part[1]) + ((double)part[2] + (double)part[3]);
         if (ep.bmax != nullptr) {
             __shared__ float wmax[NT / 64];
 #pragma unroll
