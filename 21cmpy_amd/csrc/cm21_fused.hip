@@ -336,15 +336,19 @@ int launch_lines(int mode, const float2* U, const float2* tw, const FusedParams&
     return CM21_OK;
 }
 
-// A launch whose columns/rows blocks exceed 1/kMemHeavy of its box-sum
-// blocks takes the low-occupancy build.
+// A launch whose columns/rows blocks exceed 1/kMemHeavy of its lines blocks
+// takes the low-occupancy build.
 constexpr int kMemHeavy = 2;
 
 template <int D1, int D2, int W, int N1, int N2, int L>
 int launch_radius(const RadiusRoles& a, const float2* tw, hipStream_t st) {
     const int blocks = a.n_red + a.n_thr + a.n_mem + a.n_sum;
     if (blocks <= 0) return CM21_OK;
-    if (a.n_mem * kMemHeavy > a.n_sum)
+    // the lines work of the launch: the box sums (nonlinear input), or the
+    // threshold pass (linear input, every group transforms; it carries no
+    // erfc, so it counts half)
+    const int lines = a.n_sum > 0 ? a.n_sum : a.n_thr / 2;
+    if (a.n_mem * kMemHeavy > lines)
         k_fused_radius<D1, D2, W, N1, N2, L, 4><<<blocks, W * D1, 0, st>>>(a, tw);
     else
         k_fused_radius<D1, D2, W, N1, N2, L, 8><<<blocks, W * D1, 0, st>>>(a, tw);
