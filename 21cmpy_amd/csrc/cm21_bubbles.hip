@@ -263,8 +263,10 @@ LoopWork carve(void* base, int D, int n_radii, int update) {
     char* p = (char*)base;
     LoopWork w{};
     size_t off = 0;
+    // the spectrum: the r2c layout (stride D/2+1) for the paint path, the
+    // fused transform's b-major layout with padded rows otherwise
     w.spec = (float2*)(p + off);
-    off = align_up(off + sizeof(float2) * Nc, 256);
+    off = align_up(off + sizeof(float2) * (update ? Nc : Nt), 256);
     w.filt = w.inter[0] = (float2*)(p + off);
     off = align_up(off + sizeof(float2) * Nt, 256);
     // 8Nt >= 4N bytes

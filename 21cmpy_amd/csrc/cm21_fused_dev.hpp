@@ -79,9 +79,10 @@ __device__ __forceinline__ void transpose_finish(float2 (&v)[D2], const float2* 
 // Columns pass of one block: window + inverse DFT along axis 0 for the W
 // columns (b, c0..c0+W-1), b given by its slot among the surviving rows
 // (0..bpos positive, then the negative modes).  lds: W*(D/ROUNDS+1) float2.
-// S_BMAJOR: S is stored b-major, S[b][a][c] (the fused loop's own forward
-// transform writes it so: a column along a is then D rows of one contiguous
-// plane), else in the r2c layout S[a][b][c].
+// S_BMAJOR: S is stored b-major, S[b][a][c] with rows padded to
+// padded_row(D/2+1) modes (the fused loop's own forward transform writes it
+// so: a column group along a is then D aligned 128-byte pieces of one
+// contiguous plane), else in the r2c layout S[a][b][c] (stride D/2+1).
 template <int D1, int D2, int W, int ROUNDS = 1, bool S_BMAJOR = false>
 __device__ __forceinline__ void columns_body(const float2* __restrict__ S, float2* __restrict__ T, int ho,
                                              const float2* __restrict__ tw, const FusedParams& fp, float scale,
@@ -103,7 +104,7 @@ __device__ __forceinline__ void columns_body(const float2* __restrict__ S, float
         const int as = mode_of(a, D);
         const int m2 = as * as + bs * bs + c * c;
         float2 x = make_float2(0.0f, 0.0f);
-        const size_t si = S_BMAJOR ? ((size_t)b * D + a) * H + c : ((size_t)a * D + b) * H + c;
+        const size_t si = S_BMAJOR ? ((size_t)b * D + a) * padded_row(H) + c : ((size_t)a * D + b) * H + c;
         if (col_ok && m2 <= fp.m2max) x = fused_window(S[si], a, b, c, m2, D, fp, scale);
         v[n2] = x;
     }
@@ -143,7 +144,7 @@ __device__ __forceinline__ void rows_body(float2* __restrict__ T, int ho, const 
 // conj(inverse(conj x))).  Rows: plane p, in place along b, every b loaded
 // for c <= cmax, stored only where b^2 + c^2 <= m2max (the columns pass reads
 // nothing else).  Columns: T (rows layout, stride ho) along a into the
-// half spectrum S, stored b-major (S[b][a][c], stride D/2+1), only columns
+// half spectrum S, stored b-major (S[b][a][c], stride padded_row(D/2+1)), only columns
 // inside the disc and modes inside the sphere |n|^2 <= m2max (no radius
 // reads beyond it).
 template <int D1, int D2, int W>
@@ -190,7 +191,7 @@ __device__ __forceinline__ void fwd_columns_body(const float2* __restrict__ T, i
     dft<D2>(v);
     transpose_finish<D1, D2, W, 1>(v, tw, lds, [&](int jj, int k2, int k1, float2 u) {
         const int cc = c0 + jj, a = k2 + D2 * k1, as = mode_of(a, D);
-        if (cc <= cmax && as * as + bs * bs + cc * cc <= m2max) S[((size_t)b * D + a) * H + cc] = cconj(u);
+        if (cc <= cmax && as * as + bs * bs + cc * cc <= m2max) S[((size_t)b * D + a) * padded_row(H) + cc] = cconj(u);
     });
 }
 

@@ -90,15 +90,15 @@ def test_abi_version_and_argument_errors(cdll):
 
 def test_workspace_sizes(cdll):
     """Workspace queries are pure host arithmetic: they grow with D and with the
-    number of radii; without the sphere paint they cover the spectrum, the five
-    intermediates of the pipelined loop (rows padded to 16 modes) and the
+    number of radii; without the sphere paint they cover the spectrum and the
+    five intermediates of the pipelined loop (rows padded to 16 modes) and the
     mask; with it, the spectrum, the filtered spectrum, the smoothed box and
     the two uint32 distance boxes of the paint."""
     D = 256
     nc = D * D * (D // 2 + 1)
     nt = D * D * ((D // 2 + 1 + 15) // 16 * 16)
     w40 = cdll.cm21_find_bubbles_workspace_bytes(D, 40, 0)
-    assert w40 >= 8 * nc + 5 * 8 * nt + D ** 3 // 8
+    assert w40 >= 6 * 8 * nt + D ** 3 // 8
     assert cdll.cm21_find_bubbles_workspace_bytes(D, 41, 0) >= w40
     assert cdll.cm21_find_bubbles_workspace_bytes(128, 40, 0) < w40
     assert cdll.cm21_find_bubbles_workspace_bytes(D, 40, 1) >= 8 * nc + 8 * nt + 4 * D ** 3 + 2 * 4 * D ** 3
