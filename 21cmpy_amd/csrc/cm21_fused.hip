@@ -337,8 +337,16 @@ int launch_lines(int mode, const float2* U, const float2* tw, const FusedParams&
 }
 
 // A launch whose columns/rows blocks exceed 1/kMemHeavy of its lines blocks
-// takes the low-occupancy build.
+// takes the low-occupancy build.  With at least kSplitBlocks of them (eight
+// per CU of the 256, two rounds of the low-occupancy build) the launch is
+// issued as two: the columns/rows roles alone in the low-occupancy build,
+// then the other roles in the high-occupancy one.  In one launch the
+// transposes and the box sums share each CU at the occupancy of the former;
+// at the smallest radii of a 256^3 box (about 4600 columns/rows blocks) the
+// two launches take less time than the one, while below about 2000 blocks
+// (boxes up to 160^3) the second launch costs more than it saves.
 constexpr int kMemHeavy = 2;
+constexpr int kSplitBlocks = 2048;
 
 template <int D1, int D2, int W, int N1, int N2, int L>
 int launch_radius(const RadiusRoles& a, const float2* tw, hipStream_t st) {
@@ -348,10 +356,20 @@ int launch_radius(const RadiusRoles& a, const float2* tw, hipStream_t st) {
     // threshold pass (linear input, every group transforms; it carries no
     // erfc, so it counts half)
     const int lines = a.n_sum > 0 ? a.n_sum : a.n_thr / 2;
-    if (a.n_mem * kMemHeavy > lines)
+    if (a.n_mem * kMemHeavy > lines && a.n_mem >= kSplitBlocks) {
+        RadiusRoles m = a;  // the columns/rows roles
+        m.n_red = m.n_thr = m.n_sum = 0;
+        k_fused_radius<D1, D2, W, N1, N2, L, 4><<<a.n_mem, W * D1, 0, st>>>(m, tw);
+        CM21_LAUNCH_CHECK("k_fused_radius");
+        RadiusRoles r = a;  // the reduction, threshold and box-sum roles
+        r.n_mem = r.n_cols = 0;
+        const int rb = r.n_red + r.n_thr + r.n_sum;
+        if (rb > 0) k_fused_radius<D1, D2, W, N1, N2, L, 8><<<rb, W * D1, 0, st>>>(r, tw);
+    } else if (a.n_mem * kMemHeavy > lines) {
         k_fused_radius<D1, D2, W, N1, N2, L, 4><<<blocks, W * D1, 0, st>>>(a, tw);
-    else
+    } else {
         k_fused_radius<D1, D2, W, N1, N2, L, 8><<<blocks, W * D1, 0, st>>>(a, tw);
+    }
     CM21_LAUNCH_CHECK("k_fused_radius");
     return CM21_OK;
 }
