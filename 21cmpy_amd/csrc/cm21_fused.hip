@@ -209,24 +209,20 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
 // then the columns/rows blocks interleaved one for one with the box-sum
 // blocks, so that the memory-bound transposes and the VALU-bound box sums
 // share every CU.
-// Two builds differ in occupancy: WAVES = 8 lets the box-sum role (about 52
-// VGPRs) run at full occupancy while the columns/rows roles spill registers
-// to scratch; WAVES = 4 gives the columns/rows roles their registers and the
-// LDS of a one-round transpose.  The host picks the first while the box sums
-// dominate a launch and the second once the transposes do (the smallest
-// radii).
-template <int D1, int D2, int W, int N1, int N2, int L, int WAVES>
-__global__ __launch_bounds__(W* D1, WAVES) void k_fused_radius(RadiusRoles a, const float2* __restrict__ tw) {
+// The kernel asks for eight waves per SIMD, the box-sum role's full
+// occupancy (about 52 VGPRs, 19.5 KB of LDS at D = 256); the columns/rows
+// roles fit it without spilling through the re/im-staged transpose.
+template <int D1, int D2, int W, int N1, int N2, int L>
+__global__ __launch_bounds__(W* D1, 8) void k_fused_radius(RadiusRoles a, const float2* __restrict__ tw) {
     using C = LinesCore<N1, N2, L>;
     constexpr int D = D1 * D2, NT = W * D1;
     static_assert(C::D == D && C::NT == NT, "columns/rows and lines tiles must share D and the block size");
-    // the eight-wave build runs the columns/rows transposes in two rounds
-    // when their one-round tile would need more LDS than the lines pass (and
-    // so lower the occupancy of the box sums); the four-wave build has the
-    // LDS for one round
+    // the columns/rows transposes stage real and imaginary parts in turn
+    // (a float tile) when their one-round float2 tile would need more LDS
+    // than the lines pass, and so lower the occupancy of the box sums
     constexpr int kLdsLines = L * C::HP + D + C::WORDS;
-    constexpr int kRounds = (WAVES > 4 && W * (D + 1) > kLdsLines + kLdsLines / 4) ? 2 : 1;
-    constexpr int kLdsCR = W * (D / kRounds + 1);
+    constexpr int kRounds = W * (D + 1) > kLdsLines + kLdsLines / 4 ? 0 : 1;
+    constexpr int kLdsCR = kRounds == 0 ? (W * (D + 1) + 1) / 2 : W * (D + 1);
     __shared__ float2 lds[kLdsCR > kLdsLines ? kLdsCR : kLdsLines];
     float2* buf = lds;
     float2* stw = lds + L * C::HP;
@@ -336,40 +332,11 @@ int launch_lines(int mode, const float2* U, const float2* tw, const FusedParams&
     return CM21_OK;
 }
 
-// A launch whose columns/rows blocks exceed 1/kMemHeavy of its lines blocks
-// takes the low-occupancy build.  With at least kSplitBlocks of them (eight
-// per CU of the 256, two rounds of the low-occupancy build) the launch is
-// issued as two: the columns/rows roles alone in the low-occupancy build,
-// then the other roles in the high-occupancy one.  In one launch the
-// transposes and the box sums share each CU at the occupancy of the former;
-// at the smallest radii of a 256^3 box (about 4600 columns/rows blocks) the
-// two launches take less time than the one, while below about 2000 blocks
-// (boxes up to 160^3) the second launch costs more than it saves.
-constexpr int kMemHeavy = 2;
-constexpr int kSplitBlocks = 2048;
-
 template <int D1, int D2, int W, int N1, int N2, int L>
 int launch_radius(const RadiusRoles& a, const float2* tw, hipStream_t st) {
     const int blocks = a.n_red + a.n_thr + a.n_mem + a.n_sum;
     if (blocks <= 0) return CM21_OK;
-    // the lines work of the launch: the box sums (nonlinear input), or the
-    // threshold pass (linear input, every group transforms; it carries no
-    // erfc, so it counts half)
-    const int lines = a.n_sum > 0 ? a.n_sum : a.n_thr / 2;
-    if (a.n_mem * kMemHeavy > lines && a.n_mem >= kSplitBlocks) {
-        RadiusRoles m = a;  // the columns/rows roles
-        m.n_red = m.n_thr = m.n_sum = 0;
-        k_fused_radius<D1, D2, W, N1, N2, L, 4><<<a.n_mem, W * D1, 0, st>>>(m, tw);
-        CM21_LAUNCH_CHECK("k_fused_radius");
-        RadiusRoles r = a;  // the reduction, threshold and box-sum roles
-        r.n_mem = r.n_cols = 0;
-        const int rb = r.n_red + r.n_thr + r.n_sum;
-        if (rb > 0) k_fused_radius<D1, D2, W, N1, N2, L, 8><<<rb, W * D1, 0, st>>>(r, tw);
-    } else if (a.n_mem * kMemHeavy > lines) {
-        k_fused_radius<D1, D2, W, N1, N2, L, 4><<<blocks, W * D1, 0, st>>>(a, tw);
-    } else {
-        k_fused_radius<D1, D2, W, N1, N2, L, 8><<<blocks, W * D1, 0, st>>>(a, tw);
-    }
+    k_fused_radius<D1, D2, W, N1, N2, L><<<blocks, W * D1, 0, st>>>(a, tw);
     CM21_LAUNCH_CHECK("k_fused_radius");
     return CM21_OK;
 }

@@ -48,11 +48,61 @@ __device__ inline float2 fused_window(float2 x, int a, int b, int c, int m2, int
 // store(jj, k2, k1, value).  With ROUNDS = 2 the transpose runs in two halves
 // of k2 through a tile half the size (W*(D/2+1) float2 instead of W*(D+1)),
 // so that a launch sharing its LDS with the lines pass keeps its occupancy.
+//
+// With ROUNDS = 0 the whole transpose goes through a float tile of W*(D+1)
+// floats, real parts first and imaginary parts second: a quarter of the
+// one-round tile, with no more registers live than in one round (the real
+// parts read back take the place of the real parts written).
+template <int D1, int D2, int W, class Store>
+__device__ __forceinline__ void transpose_finish_reim(float2 (&v)[D2], const float2* __restrict__ tw, float* lds,
+                                                      Store store) {
+    constexpr int NT = W * D1, LD = D2 * D1 + 1;
+    constexpr int ITEMS = (W * D2 + NT - 1) / NT;
+    const int t = threadIdx.x;
+    const int j = t % W, n1 = t / W;
+#pragma unroll
+    for (int k2 = 0; k2 < D2; ++k2)
+        if (n1 * k2 != 0) v[k2] = cmul(v[k2], tw[n1 * k2]);
+#pragma unroll
+    for (int k2 = 0; k2 < D2; ++k2) lds[j * LD + k2 * D1 + n1] = v[k2].x;
+    __syncthreads();
+    float2 u[ITEMS][D1];
+#pragma unroll
+    for (int q = 0; q < ITEMS; ++q) {
+        const int it = t + q * NT;
+        const int jj = it % W, k2 = it / W;
+        if (it < W * D2) {
+#pragma unroll
+            for (int n = 0; n < D1; ++n) u[q][n].x = lds[jj * LD + k2 * D1 + n];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k2 = 0; k2 < D2; ++k2) lds[j * LD + k2 * D1 + n1] = v[k2].y;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < ITEMS; ++q) {
+        const int it = t + q * NT;
+        const int jj = it % W, k2 = it / W;
+        if (it < W * D2) {
+#pragma unroll
+            for (int n = 0; n < D1; ++n) u[q][n].y = lds[jj * LD + k2 * D1 + n];
+            dft<D1>(u[q]);
+#pragma unroll
+            for (int k1 = 0; k1 < D1; ++k1) store(jj, k2, k1, u[q][k1]);
+        }
+    }
+}
+
 template <int D1, int D2, int W, int ROUNDS, class Store>
 __device__ __forceinline__ void transpose_finish(float2 (&v)[D2], const float2* __restrict__ tw, float2* lds,
                                                  Store store) {
-    constexpr int NT = W * D1, K2R = D2 / ROUNDS, LD = K2R * D1 + 1;
-    static_assert(D2 % ROUNDS == 0, "rounds must split k2 evenly");
+    if constexpr (ROUNDS == 0) {
+        transpose_finish_reim<D1, D2, W>(v, tw, reinterpret_cast<float*>(lds), store);
+        return;
+    }
+    constexpr int NT = W * D1, K2R = D2 / (ROUNDS > 0 ? ROUNDS : 1), LD = K2R * D1 + 1;
+    static_assert(ROUNDS == 0 || D2 % ROUNDS == 0, "rounds must split k2 evenly");
     const int t = threadIdx.x;
     const int j = t % W, n1 = t / W;
 #pragma unroll
@@ -78,7 +128,8 @@ __device__ __forceinline__ void transpose_finish(float2 (&v)[D2], const float2* 
 
 // Columns pass of one block: window + inverse DFT along axis 0 for the W
 // columns (b, c0..c0+W-1), b given by its slot among the surviving rows
-// (0..bpos positive, then the negative modes).  lds: W*(D/ROUNDS+1) float2.
+// (0..bpos positive, then the negative modes).  lds: W*(D/ROUNDS+1) float2
+// (ROUNDS >= 1), or W*(D+1) floats (ROUNDS = 0).
 // S_BMAJOR: S is stored b-major, S[b][a][c] with rows padded to
 // padded_row(D/2+1) modes (the fused loop's own forward transform writes it
 // so: a column group along a is then D aligned 128-byte pieces of one
@@ -116,7 +167,7 @@ __device__ __forceinline__ void columns_body(const float2* __restrict__ S, float
 }
 
 // Rows pass of one block: inverse DFT along axis 1, in place, for the W
-// columns (p, c0..c0+W-1).  lds: W*(D/ROUNDS+1) float2.
+// columns (p, c0..c0+W-1).  lds as for the columns pass.
 template <int D1, int D2, int W, int ROUNDS = 1>
 __device__ __forceinline__ void rows_body(float2* __restrict__ T, int ho, const float2* __restrict__ tw,
                                           const FusedParams& fp, int p, int cgroup, float2* lds) {
