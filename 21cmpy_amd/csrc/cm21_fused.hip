@@ -293,20 +293,26 @@ __global__ __launch_bounds__(L* N1) void k_fwd_lines(const float* __restrict__ d
     }
 }
 
+// Rows and columns of the forward transform: the re/im-staged transpose (a
+// float tile of W*(D+1)) keeps the LDS of a block at a quarter of the
+// one-round tile; six waves per SIMD leave the registers to spare spills
+// (eight spill at D = 256 and run slower).
 template <int D1, int D2, int W>
-__global__ __launch_bounds__(W* D1) void k_fwd_rows(float2* __restrict__ T0, const float2* __restrict__ tw, int cmax,
-                                                   int m2max) {
+__global__ __launch_bounds__(W* D1, 6) void k_fwd_rows(float2* __restrict__ T0, const float2* __restrict__ tw,
+                                                      int cmax, int m2max) {
     constexpr int D = D1 * D2;
-    __shared__ float2 lds[W * (D + 1)];
-    fwd_rows_body<D1, D2, W>(T0, padded_row(D / 2 + 1), tw, cmax, m2max, blockIdx.y, blockIdx.x, lds);
+    __shared__ float lds[W * (D + 1)];
+    fwd_rows_body<D1, D2, W, 0>(T0, padded_row(D / 2 + 1), tw, cmax, m2max, blockIdx.y, blockIdx.x,
+                                reinterpret_cast<float2*>(lds));
 }
 
 template <int D1, int D2, int W>
-__global__ __launch_bounds__(W* D1) void k_fwd_columns(const float2* __restrict__ T0, float2* __restrict__ S,
-                                                      const float2* __restrict__ tw, int bm, int cmax, int m2max) {
+__global__ __launch_bounds__(W* D1, 6) void k_fwd_columns(const float2* __restrict__ T0, float2* __restrict__ S,
+                                                         const float2* __restrict__ tw, int bm, int cmax, int m2max) {
     constexpr int D = D1 * D2;
-    __shared__ float2 lds[W * (D + 1)];
-    fwd_columns_body<D1, D2, W>(T0, padded_row(D / 2 + 1), S, tw, bm, cmax, m2max, blockIdx.y, blockIdx.x, lds);
+    __shared__ float lds[W * (D + 1)];
+    fwd_columns_body<D1, D2, W, 0>(T0, padded_row(D / 2 + 1), S, tw, bm, cmax, m2max, blockIdx.y, blockIdx.x,
+                                   reinterpret_cast<float2*>(lds));
 }
 
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, size_t n) {

@@ -198,7 +198,7 @@ __device__ __forceinline__ void rows_body(float2* __restrict__ T, int ho, const 
 // half spectrum S, stored b-major (S[b][a][c], stride padded_row(D/2+1)), only columns
 // inside the disc and modes inside the sphere |n|^2 <= m2max (no radius
 // reads beyond it).
-template <int D1, int D2, int W>
+template <int D1, int D2, int W, int ROUNDS = 1>
 __device__ __forceinline__ void fwd_rows_body(float2* __restrict__ T, int ho, const float2* __restrict__ tw, int cmax,
                                               int m2max, int p, int cgroup, float2* lds) {
     constexpr int D = D1 * D2;
@@ -213,13 +213,13 @@ __device__ __forceinline__ void fwd_rows_body(float2* __restrict__ T, int ho, co
         v[n2] = c <= cmax ? cconj(T[((size_t)p * D + b) * ho + c]) : make_float2(0.0f, 0.0f);
     }
     dft<D2>(v);
-    transpose_finish<D1, D2, W, 1>(v, tw, lds, [&](int jj, int k2, int k1, float2 u) {
+    transpose_finish<D1, D2, W, ROUNDS>(v, tw, lds, [&](int jj, int k2, int k1, float2 u) {
         const int cc = c0 + jj, b = k2 + D2 * k1, bs = mode_of(b, D);
         if (cc <= cmax && bs * bs + cc * cc <= m2max) T[((size_t)p * D + b) * ho + cc] = cconj(u);
     });
 }
 
-template <int D1, int D2, int W>
+template <int D1, int D2, int W, int ROUNDS = 1>
 __device__ __forceinline__ void fwd_columns_body(const float2* __restrict__ T, int ho, float2* __restrict__ S,
                                                  const float2* __restrict__ tw, int bm, int cmax, int m2max, int slot,
                                                  int cgroup, float2* lds) {
@@ -240,7 +240,7 @@ __device__ __forceinline__ void fwd_columns_body(const float2* __restrict__ T, i
         v[n2] = col_ok ? cconj(T[((size_t)a * D + b) * ho + c]) : make_float2(0.0f, 0.0f);
     }
     dft<D2>(v);
-    transpose_finish<D1, D2, W, 1>(v, tw, lds, [&](int jj, int k2, int k1, float2 u) {
+    transpose_finish<D1, D2, W, ROUNDS>(v, tw, lds, [&](int jj, int k2, int k1, float2 u) {
         const int cc = c0 + jj, a = k2 + D2 * k1, as = mode_of(a, D);
         if (cc <= cmax && as * as + bs * bs + cc * cc <= m2max) S[((size_t)b * D + a) * padded_row(H) + cc] = cconj(u);
     });
