@@ -45,14 +45,16 @@ __device__ inline float2 fused_window(float2 x, int a, int b, int c, int m2, int
 // (j, n1) holds the D2-point DFT outputs v[k2] of column j; they are
 // twiddled by w_D^(n1 k2), transposed through LDS, and the D1-point DFTs over
 // n1 run per (column, k2) item, which hands element k2 + D2*k1 of column jj to
-// store(jj, k2, k1, value).  With ROUNDS = 2 the transpose runs in two halves
-// of k2 through a tile half the size (W*(D/2+1) float2 instead of W*(D+1)),
-// so that a launch sharing its LDS with the lines pass keeps its occupancy.
+// store(jj, k2, k1, value).  With ROUNDS = 1 the transpose goes through a
+// float2 tile of W*(D+1); with ROUNDS = 2 in two halves of k2 through a tile
+// half that size, which keeps the unwritten half of v live (and spills at
+// eight waves per SIMD).
 //
 // With ROUNDS = 0 the whole transpose goes through a float tile of W*(D+1)
-// floats, real parts first and imaginary parts second: a quarter of the
+// floats, real parts first and imaginary parts second: half the bytes of the
 // one-round tile, with no more registers live than in one round (the real
-// parts read back take the place of the real parts written).
+// parts read back take the place of the real parts written).  The radius
+// kernel and the forward transform use it.
 template <int D1, int D2, int W, class Store>
 __device__ __forceinline__ void transpose_finish_reim(float2 (&v)[D2], const float2* __restrict__ tw, float* lds,
                                                       Store store) {
