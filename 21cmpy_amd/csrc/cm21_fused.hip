@@ -294,7 +294,7 @@ __global__ __launch_bounds__(L* N1) void k_fwd_lines(const float* __restrict__ d
 }
 
 // Rows and columns of the forward transform: the re/im-staged transpose (a
-// float tile of W*(D+1)) keeps the LDS of a block at a quarter of the
+// float tile of W*(D+1)) keeps the LDS of a block at half the
 // one-round tile; six waves per SIMD leave the registers to spare spills
 // (eight spill at D = 256 and run slower).
 template <int D1, int D2, int W>
