@@ -68,26 +68,43 @@ struct GrfParams {
 
 constexpr int kMaxTable = 2048;
 
+// Where to start looking for k in a table: the index of k if the table were
+// spaced uniformly in log k between its end points (the P(k) tables are),
+// from log2 K[0] and (n-1) / log2(K[n-1]/K[0]); any table works, the search
+// then only walks further.
+struct TableGuess {
+    float l0, inv;
+};
+
+__device__ inline TableGuess table_guess(const float* K, int n) {
+    TableGuess g{0.0f, 0.0f};
+    if (K[0] > 0.0f && K[n - 1] > K[0]) {
+        g.l0 = __log2f(K[0]);
+        g.inv = (float)(n - 1) / (__log2f(K[n - 1]) - g.l0);
+    }
+    return g;
+}
+
 // P(k) from the table by the rule of initialize.cu:87-90: i is the first
 // index with K[i] >= k and P is interpolated linearly between i-1 and i in
-// float32, as init_kernel does; modes outside [K[0], K[n-1]] get 0.
-__device__ inline float table_pk(float k, const float* K, const float* Pt, int n) {
+// float32, as init_kernel does; modes outside [K[0], K[n-1]] get 0.  The
+// index is found from the log-uniform guess by stepping down while K[i-1] >= k
+// and up while K[i] < k: the same first index as a bisection, in one or two
+// steps on a log-spaced table.
+__device__ inline float table_pk(float k, const float* K, const float* Pt, int n, const TableGuess& tg) {
     if (!(k >= K[0]) || k > K[n - 1]) return 0.0f;
-    int lo = 0, hi = n - 1;  // K[lo] <= k <= K[hi]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (K[mid] >= k)
-            hi = mid;
-        else
-            lo = mid;
-    }
+    const float x = (__log2f(k) - tg.l0) * tg.inv + 1.0f;
+    int hi = (int)fminf(fmaxf(x, 1.0f), (float)(n - 1));  // in [1, n-1]
+    while (hi > 1 && K[hi - 1] >= k) --hi;
+    while (K[hi] < k) ++hi;  // stops at n-1 at the latest: K[n-1] >= k
+    const int lo = hi - 1;
     if (K[lo] >= k) return Pt[lo];  // k == K[0]
     return Pt[lo] + (Pt[hi] - Pt[lo]) * ((k - K[lo]) / (K[hi] - K[lo]));
 }
 
 // One mode: its Philox draw, amplitude and Hermitian placement.
 __device__ __forceinline__ void grf_mode(float2* __restrict__ spec, int D, int a, int b, int c, const GrfParams& gp,
-                                         const float* sK, const float* sP) {
+                                         const float* sK, const float* sP, const TableGuess& tg) {
     const int H = D / 2 + 1;
     const size_t idx = ((size_t)a * D + b) * H + c;
     if (a == 0 && b == 0 && c == 0) {
@@ -112,7 +129,7 @@ __device__ __forceinline__ void grf_mode(float2* __restrict__ spec, int D, int a
     const float k = gp.dkf * sqrtf((float)(na * na + nb * nb + c * c));
     float amp;
     if (gp.ntab > 0) {
-        amp = sqrtf(table_pk(k, sK, sP, gp.ntab) * gp.inv_volume);
+        amp = sqrtf(table_pk(k, sK, sP, gp.ntab, tg) * gp.inv_volume);
     } else {
         const float x = k * gp.inv_k_cut;
         // k^n as exp2(n log2 k) (k > 0 here): about 1e-6 relative, a third of
@@ -143,13 +160,14 @@ __global__ __launch_bounds__(kModeBlockX* kModeBlockY) void k_grf(float2* __rest
         }
         __syncthreads();
     }
+    const TableGuess tg = gp.ntab > 0 ? table_guess(sK, gp.ntab) : TableGuess{0.0f, 0.0f};
     const int H = D / 2 + 1;
     const int a = blockIdx.z;
     const int b0 = blockIdx.y * kGrfRows;
     const int n = (min(D, b0 + kGrfRows) - b0) * H;  // the block's modes, rows b0.. of plane a
     for (int q = threadIdx.y * kModeBlockX + threadIdx.x; q < n; q += kModeBlockX * kModeBlockY) {
         const int rb = q / H;
-        grf_mode(spec, D, a, b0 + rb, q - rb * H, gp, sK, sP);
+        grf_mode(spec, D, a, b0 + rb, q - rb * H, gp, sK, sP, tg);
     }
 }
 

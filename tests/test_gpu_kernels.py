@@ -133,6 +133,13 @@ def test_grf_table_matches_oracle(gpu):
     want = grf.grf_half_spectrum(16, 40.0, 1, pk_table=(K, Pk))
     _within_grf_tolerance(got, want)
     assert np.any(got == 0) and np.any(got != 0)
+    # a table spaced linearly in k: the search starts from a log-uniform
+    # guess and walks further to the same index
+    K = np.linspace(0.01, 3.0, 300).astype(np.float32)
+    Pk = (100.0 / (1.0 + (K / 0.3) ** 2)).astype(np.float32)
+    got = g.grf_init_table(16, 40.0, K, Pk, 2).cpu().numpy()
+    want = grf.grf_half_spectrum(16, 40.0, 2, pk_table=(K, Pk))
+    _within_grf_tolerance(got, want)
 
 
 def test_init_pipeline_planck_table_matches_oracle(gpu):
