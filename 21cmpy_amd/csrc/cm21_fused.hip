@@ -295,8 +295,8 @@ __global__ __launch_bounds__(L* N1) void k_fwd_lines(const float* __restrict__ d
 
 // Rows and columns of the forward transform: the re/im-staged transpose (a
 // float tile of W*(D+1)) keeps the LDS of a block at half the
-// one-round tile; six waves per SIMD leave the registers to spare spills
-// (eight spill at D = 256 and run slower).
+// one-round tile; at six waves per SIMD they need no scratch (at eight they
+// spill a few registers at D = 256 and run slower).
 template <int D1, int D2, int W>
 __global__ __launch_bounds__(W* D1, 6) void k_fwd_rows(float2* __restrict__ T0, const float2* __restrict__ tw,
                                                       int cmax, int m2max) {
