@@ -12,7 +12,7 @@ order of the calls.  Training minimises the mean binary cross entropy
 against the plain method's ionized field with AdamW.
 
 ``SurrogatePipeline`` serves requests with the network and checks a sample of
-them against the plain method (the fallback and the judge): when a sampled
+them against the plain method (the fallback and the reference answer): when a sampled
 box misses the tolerance (|dx_HI| or the P21(k) error) the batch is
 recomputed with the plain method.
 

@@ -476,7 +476,7 @@ def main():
                          "copy (to calibrate FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(independent boxes shard across GPUs with no exchange; see the design notes)", file=sys.stderr)
         sys.exit(2)
 
