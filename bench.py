@@ -631,7 +631,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_gbs": (traffic / (dom_ms * 1e-3) / 1e9) if traffic else None,
                      "traffic_frac": (traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
-                     "alg_bytes_per_launch": ab[dom], "ms_per_launch": dom_ms},
+                     "alg_bytes_per_launch": ab[dom], "ms_per_launch": dom_ms,
+                     "note": "achieved credits the unpruned algorithm's bytes; the pruned launch moves "
+                             "`traffic` bytes (traffic_frac of peak) and is bound by VALU issue (valu_roofline)"},
         # what bounds the kernel: VALU issue (counter-measured instructions per
         # launch over this run's launch time)
         "valu_roofline": ({"achieved": valu / (dom_ms * 1e-3) / 1e9, "peak": VALU_PEAK_G,
