@@ -121,7 +121,7 @@ __device__ __forceinline__ void lines_block(const float2* __restrict__ U, const 
     C::store(tmp, buf);
     if (MODE != kLineSum && t < WORDS) smask[t] = mword;
     __syncthreads();
-    C::transform(buf, stw);
+    C::transform(buf, stw, fp.cmax + 1);
     double acc = 0.0;
     float vmax = -INFINITY;
     const int lane = t & 63;

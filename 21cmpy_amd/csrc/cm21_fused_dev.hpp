@@ -289,21 +289,36 @@ struct LinesCore {
 
     // In: buf holds the half lines (after a barrier).  Out: buf holds the real
     // lines, line i at ((float*)buf)[i * 2 * HP + r], followed by a barrier.
-    __device__ static __forceinline__ void transform(float2* buf, const float2* stw) {
+    // cl: the modes c >= cl of every line are zero (buf holds zeros there).
+    // With cl <= N1 only the first and the last of a thread's N2 inputs Z[k]
+    // can be non-zero (k = n1 has X[k]; k = n1 + N1 (N2-1) has conj X[N-k]),
+    // and the N2-point DFT reduces to V[k2] = Z0 + Z_last w_N2^((N2-1) k2).
+    __device__ static __forceinline__ void transform(float2* buf, const float2* stw, int cl = H) {
         const int t = threadIdx.x;
         {
             const int i = t % L, n1 = t / L;  // lanes walk lines: distinct banks
             float2 v[N2];
+            if (cl <= N1) {
+                const int k0 = n1, k1 = n1 + N1 * (N2 - 1);
+                const float2 X0 = buf[i * HP + k0];
+                const float2 z0 = cadd(X0, times_i(k0 == 0 ? X0 : cmul(X0, stw[k0])));
+                const float2 Xm = cconj(buf[i * HP + (N - k1)]);
+                const float2 z1 = csub(Xm, times_i(cmul(Xm, stw[k1])));
 #pragma unroll
-            for (int n2 = 0; n2 < N2; ++n2) {
-                const int k = n1 + N1 * n2;
-                const float2 X = buf[i * HP + k];
-                const float2 Xm = cconj(buf[i * HP + (N - k)]);
-                const float2 s = cadd(X, Xm);
-                const float2 d = csub(X, Xm);
-                v[n2] = cadd(s, times_i(k == 0 ? d : cmul(d, stw[k])));
+                for (int k2 = 0; k2 < N2; ++k2)
+                    v[k2] = k2 == 0 ? cadd(z0, z1) : cadd(z0, cmul(z1, twiddle((long)(N2 - 1) * k2, N2)));
+            } else {
+#pragma unroll
+                for (int n2 = 0; n2 < N2; ++n2) {
+                    const int k = n1 + N1 * n2;
+                    const float2 X = buf[i * HP + k];
+                    const float2 Xm = cconj(buf[i * HP + (N - k)]);
+                    const float2 s = cadd(X, Xm);
+                    const float2 d = csub(X, Xm);
+                    v[n2] = cadd(s, times_i(k == 0 ? d : cmul(d, stw[k])));
+                }
+                dft<N2>(v);
             }
-            dft<N2>(v);
             __syncthreads();
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2)
