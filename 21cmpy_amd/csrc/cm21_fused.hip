@@ -204,11 +204,16 @@ __global__ __launch_bounds__(L* N1) void k_fused_lines(const float2* __restrict_
 // what earlier launches wrote and writes buffers no other role of the launch
 // reads or writes, so the roles need no ordering among themselves; in-order
 // launches on one stream order the radii.
-// Block order: the reduction, then the threshold blocks (block i takes the
-// i-th listed line group; those past the end of the list return at once),
-// then the columns/rows blocks interleaved one for one with the box-sum
-// blocks, so that the memory-bound transposes and the VALU-bound box sums
-// share every CU.
+// Block order: the reduction, then for nonlinear input the box-sum blocks,
+// the columns/rows blocks and the threshold blocks (block i of these takes
+// the i-th listed line group; those past the end of the list return at
+// once), for linear input the threshold blocks and then the columns/rows
+// blocks.  The VALU-bound box sums fill every CU first and the
+// memory-bound transposes take the slots they free; against the columns/rows
+// blocks interleaved one for one with the box sums after the threshold
+// blocks, 578 -> 603 boxes/s (literal) and 450 -> 457 (partly ionized) at
+// 256^3; thresholds first stay better for linear input (128^3: 2530
+// against 2399 boxes/s with thresholds last).
 // The kernel asks for eight waves per SIMD, the box-sum role's full
 // occupancy (about 52 VGPRs, 19.5 KB of LDS at D = 256); the columns/rows
 // roles fit it without spilling through the re/im-staged transpose.
@@ -233,7 +238,11 @@ __global__ __launch_bounds__(W* D1, 8) void k_fused_radius(RadiusRoles a, const 
         return;
     }
     b -= a.n_red;
-    if (b < a.n_thr) {
+    // nonlinear input: box sums, transposes, thresholds; linear input (no
+    // box sums): thresholds, transposes
+    const int thr0 = a.n_sum > 0 ? a.n_sum + a.n_mem : 0;
+    if (b >= thr0 && b < thr0 + a.n_thr) {
+        b -= thr0;
         if (a.ep_thr.active != nullptr) {
             if (b >= *a.ep_thr.nactive) return;
             b = a.ep_thr.active[b];
@@ -241,17 +250,9 @@ __global__ __launch_bounds__(W* D1, 8) void k_fused_radius(RadiusRoles a, const 
         lines_block<N1, N2, L, kLineThreshold>(a.Uthr, tw, a.fp_thr, a.ep_thr, b, buf, stw, smask);
         return;
     }
-    b -= a.n_thr;
-    const int pairs = a.n_mem < a.n_sum ? a.n_mem : a.n_sum;
-    bool mem;
-    int i;
-    if (b < 2 * pairs) {
-        mem = (b & 1) == 0;
-        i = b >> 1;
-    } else {
-        mem = a.n_mem > a.n_sum;
-        i = pairs + (b - 2 * pairs);
-    }
+    if (b >= thr0 + a.n_thr) b -= a.n_thr;
+    const bool mem = b >= a.n_sum;
+    int i = mem ? b - a.n_sum : b;
     if (!mem) {
         lines_block<N1, N2, L, kLineSum>(a.Usum, tw, a.fp_sum, a.ep_sum, i, buf, stw, smask);
     } else if (i < a.n_cols) {
